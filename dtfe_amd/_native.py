@@ -1,0 +1,161 @@
+"""ctypes binding of the C ABI in include/dtfe_hip.h (libdtfe_hip.so).
+
+The library must have been built (``python -m dtfe_amd.build`` or
+``__graft_entry__.build()``); loading fails loudly otherwise — there is no
+CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdtfe_hip.so")
+
+DTFE_OK = 0
+DTFE_ERR_ARGUMENT = 1
+DTFE_ERR_HIP = 2
+DTFE_ERR_NO_MEMORY = 3
+DTFE_ERR_TOO_FEW_VERTICES = 4
+DTFE_ERR_WALK = 5
+
+DTFE_F32 = 0
+DTFE_F64 = 1
+
+FIELD_DENSITY = 1 << 0
+FIELD_VELOCITY = 1 << 1
+FIELD_VELOCITY_GRADIENT = 1 << 2
+FIELD_VELOCITY_DIVERGENCE = 1 << 3
+FIELD_VELOCITY_SHEAR = 1 << 4
+FIELD_VELOCITY_VORTICITY = 1 << 5
+FIELD_VELOCITY_STD = 1 << 6
+FIELD_SCALAR = 1 << 7
+FIELD_SCALAR_GRADIENT = 1 << 8
+
+COMPAT_VALUE_BUG = 1 << 0
+AVERAGE_FINITE_ONLY = 1 << 1
+
+SAMPLE_GRID = 0
+SAMPLE_CONE = 1
+SAMPLE_POINTS = 2
+
+METHOD_CENTRE = 0
+METHOD_MC = 2
+METHOD_EQUI = 3
+
+# Every symbol declared in include/dtfe_hip.h
+EXPORTED = [
+    "dtfe_hip_create", "dtfe_hip_destroy", "dtfe_hip_set_stream", "dtfe_hip_synchronize",
+    "dtfe_hip_last_error", "dtfe_hip_vertex_density", "dtfe_hip_sample", "dtfe_hip_deposit",
+    "dtfe_hip_version",
+]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("n_vertices", C.c_int64), ("n_cells", C.c_int64), ("pos", C.c_void_p),
+                ("cells", C.c_void_p), ("neighbors", C.c_void_p)]
+
+
+class VertexFields(C.Structure):
+    _fields_ = [("dtype", C.c_int), ("density", C.c_void_p), ("velocity", C.c_void_p),
+                ("scalar", C.c_void_p), ("n_scalar", C.c_int)]
+
+
+class Sampling(C.Structure):
+    _fields_ = [("kind", C.c_int), ("box", C.c_double * 6), ("n", C.c_int64 * 3),
+                ("cone", C.c_double * 6), ("origin", C.c_double * 3), ("n_points", C.c_int64),
+                ("points", C.c_void_p), ("deltas", C.c_void_p)]
+
+
+class MC(C.Structure):
+    _fields_ = [("method", C.c_int), ("samples", C.c_int), ("seed", C.c_uint64),
+                ("part_no", C.c_int), ("arg_order", C.c_int)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [("density", C.c_void_p), ("velocity", C.c_void_p), ("velocity_gradient", C.c_void_p),
+                ("velocity_divergence", C.c_void_p), ("velocity_shear", C.c_void_p),
+                ("velocity_vorticity", C.c_void_p), ("velocity_std", C.c_void_p),
+                ("scalar", C.c_void_p), ("scalar_gradient", C.c_void_p),
+                ("located_cell", C.c_void_p), ("n_written", C.c_int64)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+class DTFEError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dtfe_hip error {code}: {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Loads libdtfe_hip.so (without touching the GPU)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m dtfe_amd.build` "
+                               "(the DTFE path has no CPU fallback)")
+        lib = C.CDLL(path)
+        vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int
+        lib.dtfe_hip_create.argtypes = [i32, C.POINTER(vp)]
+        lib.dtfe_hip_destroy.argtypes = [vp]
+        lib.dtfe_hip_set_stream.argtypes = [vp, vp]
+        lib.dtfe_hip_synchronize.argtypes = [vp]
+        lib.dtfe_hip_last_error.argtypes = [vp]
+        lib.dtfe_hip_last_error.restype = C.c_char_p
+        lib.dtfe_hip_version.restype = C.c_char_p
+        lib.dtfe_hip_vertex_density.argtypes = [vp, C.POINTER(Mesh), i32, vp, C.c_double, vp]
+        lib.dtfe_hip_sample.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), C.c_uint,
+                                        C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Outputs)]
+        lib.dtfe_hip_deposit.argtypes = [vp, i32, i64, vp, i32, vp, vp, C.POINTER(C.c_double * 6),
+                                         C.POINTER(C.c_int64 * 3), C.c_double, C.c_uint, C.POINTER(Outputs)]
+        lib.dtfe_hip_last_deferred.argtypes = [vp]
+        lib.dtfe_hip_walk_failures.argtypes = [vp, C.POINTER(C.c_int)]
+        for name in EXPORTED:
+            if name not in ("dtfe_hip_last_error", "dtfe_hip_version"):
+                getattr(lib, name).restype = C.c_int
+        _lib = lib
+        return lib
+
+
+class Context:
+    """One HIP device context of the library (its own stream by default)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.dtfe_hip_create(int(device), C.byref(h))
+        if rc != DTFE_OK:
+            raise DTFEError(rc, f"dtfe_hip_create(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def check(self, rc: int, allow=()):
+        if rc != DTFE_OK and rc not in allow:
+            raise DTFEError(rc, self.lib.dtfe_hip_last_error(self.h).decode())
+        return rc
+
+    def set_stream(self, stream_ptr):
+        self.check(self.lib.dtfe_hip_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        self.check(self.lib.dtfe_hip_synchronize(self.h))
+
+    def last_deferred(self) -> int:
+        return int(self.lib.dtfe_hip_last_deferred(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dtfe_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,218 @@
+"""Tensor-level interface to the gfx950 DTFE kernels.
+
+PyTorch provides device memory and the stream; every computation runs in
+libdtfe_hip.so. Field names follow the reference's ``Field`` flags
+(src/user_options.h:48-97) and output layouts follow ``Quantities``
+(src/quantities.h:56-76): one array per field, rows in grid order
+(x slowest, z fastest), components per row as the reference's Pvector.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Dict, Iterable, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .mesh import Mesh
+
+FIELD_BITS = {
+    "density": N.FIELD_DENSITY,
+    "velocity": N.FIELD_VELOCITY,
+    "velocity_gradient": N.FIELD_VELOCITY_GRADIENT,
+    "velocity_divergence": N.FIELD_VELOCITY_DIVERGENCE,
+    "velocity_shear": N.FIELD_VELOCITY_SHEAR,
+    "velocity_vorticity": N.FIELD_VELOCITY_VORTICITY,
+    "velocity_std": N.FIELD_VELOCITY_STD,
+    "scalar": N.FIELD_SCALAR,
+    "scalar_gradient": N.FIELD_SCALAR_GRADIENT,
+}
+
+
+def components(name: str, n_scalar: int) -> int:
+    return {"density": 1, "velocity": 3, "velocity_gradient": 9, "velocity_divergence": 1,
+            "velocity_shear": 5, "velocity_vorticity": 3, "velocity_std": 1,
+            "scalar": n_scalar, "scalar_gradient": 3 * n_scalar}[name]
+
+
+_contexts: Dict[int, N.Context] = {}
+
+
+def context(device: torch.device) -> N.Context:
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    ctx = _contexts.get(idx)
+    if ctx is None:
+        ctx = N.Context(idx)
+        _contexts[idx] = ctx
+    ctx.set_stream(torch.cuda.current_stream(device).cuda_stream)
+    return ctx
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return N.DTFE_F32
+    if dtype == torch.float64:
+        return N.DTFE_F64
+    raise ValueError("field dtype must be float32 or float64")
+
+
+@dataclass
+class DeviceMesh:
+    pos: torch.Tensor    # [nv, 3] float64
+    cells: torch.Tensor  # [nc, 4] int32
+    nbrs: torch.Tensor   # [nc, 4] int32
+
+    @staticmethod
+    def from_mesh(mesh: Mesh, device="cuda") -> "DeviceMesh":
+        dev = torch.device(device)
+        return DeviceMesh(pos=torch.from_numpy(np.ascontiguousarray(mesh.pos)).to(dev),
+                          cells=torch.from_numpy(np.ascontiguousarray(mesh.cells)).to(dev),
+                          nbrs=torch.from_numpy(np.ascontiguousarray(mesh.nbrs)).to(dev))
+
+    @property
+    def device(self):
+        return self.pos.device
+
+    def c(self) -> N.Mesh:
+        for t in (self.pos, self.cells, self.nbrs):
+            assert t.is_contiguous() and t.is_cuda
+        assert self.pos.dtype == torch.float64 and self.cells.dtype == torch.int32
+        return N.Mesh(self.pos.shape[0], self.cells.shape[0], self.pos.data_ptr(),
+                      self.cells.data_ptr(), self.nbrs.data_ptr())
+
+
+def vertex_density(mesh: DeviceMesh, weight: torch.Tensor, rho_bar: float) -> torch.Tensor:
+    """DTFE vertex densities rho_v = 4 w_v / (rho_bar * sum of incident volumes)."""
+    ctx = context(mesh.device)
+    weight = weight.contiguous()
+    out = torch.empty_like(weight)
+    cm = mesh.c()
+    rc = ctx.lib.dtfe_hip_vertex_density(ctx.h, C.byref(cm), _dtype_code(weight.dtype), _ptr(weight),
+                                         float(rho_bar), _ptr(out))
+    ctx.check(rc, allow=(N.DTFE_ERR_TOO_FEW_VERTICES,))
+    return out
+
+
+@dataclass
+class GridSpec:
+    box: Sequence[float]
+    n: Sequence[int]
+
+
+@dataclass
+class ConeSpec:
+    cone: Sequence[float]     # r_min, r_max, theta_min, theta_max, psi_min, psi_max (degrees)
+    n: Sequence[int]
+    origin: Sequence[float]
+
+
+@dataclass
+class PointsSpec:
+    points: torch.Tensor               # [n, 3] float64 (device)
+    deltas: Optional[torch.Tensor]     # [n, 3] float64 (device) or None
+    box: Sequence[float]               # region filter
+
+
+def sample(mesh: DeviceMesh, rho: Optional[torch.Tensor], vel: Optional[torch.Tensor],
+           scalar: Optional[torch.Tensor], fields: Iterable[str], spec, method: int = 0,
+           samples: int = 1, seed: int = 0, part_no: int = -1, arg_order: int = 0,
+           value_bug: bool = True, finite_only: bool = False, want_cells: bool = False,
+           dtype: Optional[torch.dtype] = None) -> Dict[str, torch.Tensor]:
+    """Samples the DTFE fields on a grid, redshift cone or user points."""
+    fields = list(fields)
+    ref = rho if rho is not None else (vel if vel is not None else scalar)
+    if dtype is None:
+        dtype = ref.dtype if ref is not None else torch.float32
+    dev = mesh.device
+    ctx = context(dev)
+    n_scalar = 0 if scalar is None else int(scalar.shape[1])
+    vf = N.VertexFields(_dtype_code(dtype), _ptr(rho), _ptr(vel), _ptr(scalar), n_scalar)
+    sp = N.Sampling()
+    keep = []
+    if isinstance(spec, GridSpec):
+        sp.kind = N.SAMPLE_GRID
+        sp.box[:] = [float(x) for x in spec.box]
+        sp.n[:] = [int(x) for x in spec.n]
+        n_rows = int(np.prod(spec.n))
+    elif isinstance(spec, ConeSpec):
+        sp.kind = N.SAMPLE_CONE
+        sp.cone[:] = [float(x) for x in spec.cone]
+        sp.n[:] = [int(x) for x in spec.n]
+        sp.origin[:] = [float(x) for x in spec.origin]
+        n_rows = int(np.prod(spec.n))
+    elif isinstance(spec, PointsSpec):
+        sp.kind = N.SAMPLE_POINTS
+        sp.box[:] = [float(x) for x in spec.box]
+        pts = spec.points.contiguous()
+        keep.append(pts)
+        sp.n_points = pts.shape[0]
+        sp.points = pts.data_ptr()
+        if spec.deltas is not None:
+            dl = spec.deltas.contiguous()
+            keep.append(dl)
+            sp.deltas = dl.data_ptr()
+        n_rows = int(pts.shape[0])
+    else:
+        raise TypeError("spec must be GridSpec, ConeSpec or PointsSpec")
+    mc = N.MC(int(method), int(samples), int(seed) & ((1 << 64) - 1), int(part_no), int(arg_order))
+    out = {}
+    bits = 0
+    for f in fields:
+        bits |= FIELD_BITS[f]
+        k = components(f, n_scalar)
+        out[f] = torch.empty((n_rows, k) if k != 1 else (n_rows,), dtype=dtype, device=dev)
+    cells = torch.empty(n_rows, dtype=torch.int32, device=dev) if want_cells else None
+    o = N.Outputs()
+    for f in fields:
+        setattr(o, f, out[f].data_ptr())
+    if cells is not None:
+        o.located_cell = cells.data_ptr()
+    flags = (N.COMPAT_VALUE_BUG if value_bug else 0) | (N.AVERAGE_FINITE_ONLY if finite_only else 0)
+    cm = mesh.c()
+    rc = ctx.lib.dtfe_hip_sample(ctx.h, C.byref(cm), C.byref(vf), bits, C.byref(sp), C.byref(mc), flags,
+                                 C.byref(o))
+    ctx.check(rc, allow=(N.DTFE_ERR_TOO_FEW_VERTICES,))
+    nw = int(o.n_written)
+    if nw != n_rows:
+        out = {k: v[:nw] for k, v in out.items()}
+        if cells is not None:
+            cells = cells[:nw]
+    if cells is not None:
+        out["located_cell"] = cells
+    return out
+
+
+def deposit(scheme: str, pos: torch.Tensor, weight: torch.Tensor, vel: Optional[torch.Tensor], box, n,
+            rho_bar: float, fields=("density", "velocity")) -> Dict[str, torch.Tensor]:
+    """CIC ('cic') or TSC ('tsc') mass and velocity deposit on a regular grid."""
+    dev = pos.device
+    ctx = context(dev)
+    dtype = weight.dtype
+    ncell = int(np.prod(n))
+    out = {}
+    o = N.Outputs()
+    bits = 0
+    if "density" in fields:
+        out["density"] = torch.empty(ncell, dtype=dtype, device=dev)
+        o.density = out["density"].data_ptr()
+        bits |= N.FIELD_DENSITY
+    if "velocity" in fields:
+        out["velocity"] = torch.empty((ncell, 3), dtype=dtype, device=dev)
+        o.velocity = out["velocity"].data_ptr()
+        bits |= N.FIELD_VELOCITY
+    b = (C.c_double * 6)(*[float(x) for x in box])
+    nn = (C.c_int64 * 3)(*[int(x) for x in n])
+    pos = pos.contiguous()
+    weight = weight.contiguous()
+    vel = None if vel is None else vel.contiguous()
+    rc = ctx.lib.dtfe_hip_deposit(ctx.h, 0 if scheme == "cic" else 1, pos.shape[0], _ptr(pos),
+                                  _dtype_code(dtype), _ptr(weight), _ptr(vel), C.byref(b), C.byref(nn),
+                                  float(rho_bar), bits, C.byref(o))
+    ctx.check(rc)
+    return out

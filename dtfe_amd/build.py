@@ -1,0 +1,53 @@
+"""Builds the native libraries of the package in-tree.
+
+``libdtfe_hip.so``: the gfx950 kernels and C ABI (hipcc).
+The compile flags keep packed-FP32 instructions off, honour
+``#pragma clang fp contract`` and use hardware floating-point atomics.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libdtfe_hip.so")
+
+HIP_SOURCES = ["dtfe_hip.hip"]
+HIP_HEADERS = ["predicates.h", "mt19937.h"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-munsafe-fp-atomics", "-ffp-contract=fast-honor-pragmas",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build dtfe_amd")
+
+
+def build_hip(verbose: bool = True) -> str:
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp", *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def build_all(verbose: bool = True) -> None:
+    build_hip(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,1577 @@
+// MI355X (gfx950) kernels and C ABI of the DTFE interpolation path.
+//
+// Reference path replaced (file:line relative to the reference root):
+//   vertexDensity ............ src/CGAL_triangulation/triangulation.cpp:200-279
+//   positionMatrix/matrixInverse/densityGrad/...
+//                              src/CGAL_triangulation/unaveraged_interpolation.cc:34-173,
+//                              src/CGAL_triangulation/triangulation_miscellaneous.cc:42-96
+//   interpolateGrid / RedshiftCone / UserSampling (cell centre)
+//                              src/CGAL_triangulation/unaveraged_interpolation.cc:209-691
+//   ..._averaged_2 (Monte Carlo), interpolateGrid_averaged_3 (equidistant)
+//                              src/CGAL_triangulation/averaged_interpolation_2.cc:35-866
+//   computeDivergenceShearVorticity  src/DTFE.cpp:394-475 (fused epilogue)
+//   CIC / TSC deposits ....... src/CIC_interpolation.cc:67-273, src/TSC_interpolation.cc:67-254
+//
+// Layout in HBM: positions [nv][3] FP64, tetrahedra and neighbours as int4
+// records, vertex fields [nv][k] in the field precision. One thread owns one
+// output row (grid cell, cone bin or user point) and all of its samples.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dtfe_hip.h"
+#include "mt19937.h"
+#include "predicates.h"
+
+namespace dtfe {
+
+constexpr int kBlock = 256;
+constexpr int kMaxScalars = 4;
+constexpr int kMaxWalkSteps = 1 << 20;
+
+// ----------------------------------------------------------------------------
+// small helpers
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ d3 load_pos(const double *__restrict__ pos, int v) {
+    const double *q = pos + 3 * (int64_t)v;
+    return d3{q[0], q[1], q[2]};
+}
+
+__device__ __forceinline__ d3 sub(d3 a, d3 b) { return d3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+__device__ __forceinline__ d3 cross(d3 a, d3 b) {
+    return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ __forceinline__ d3 abs_cross(d3 a, d3 b) {
+    return d3{fabs(a.y * b.z) + fabs(a.z * b.y), fabs(a.z * b.x) + fabs(a.x * b.z),
+              fabs(a.x * b.y) + fabs(a.y * b.x)};
+}
+
+__device__ __forceinline__ d3 dabs(d3 a) { return d3{fabs(a.x), fabs(a.y), fabs(a.z)}; }
+
+// a*b + c with both roundings (the reference's sample coordinates are
+// computed without fused multiply-add).
+__device__ __forceinline__ double mul_add_rn(double a, double b, double c) {
+#pragma clang fp contract(off)
+    double t = a * b;
+    return t + c;
+}
+
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// c + a*b with both roundings
+__device__ __forceinline__ double add_mul_rn(double c, double a, double b) {
+#pragma clang fp contract(off)
+    double t = a * b;
+    return c + t;
+}
+
+// ----------------------------------------------------------------------------
+// point location: visibility walk with certified FP64 signs
+// ----------------------------------------------------------------------------
+// Face i is the face opposite vertex i; O_i(p) = orient(face_i, p) is the
+// determinant of the tetrahedron with vertex i replaced by p, so the
+// barycentric coordinates are O_i / sum(O).
+__device__ __forceinline__ int face_sos(int i, const d3 X[4], d3 p) {
+    switch (i) {
+        case 0: return orient_sos_slow(X[1], X[3], X[2], p);
+        case 1: return orient_sos_slow(X[0], X[2], X[3], p);
+        case 2: return orient_sos_slow(X[0], X[3], X[1], p);
+        default: return orient_sos_slow(X[0], X[1], X[2], p);
+    }
+}
+
+struct Located {
+    int t;        // cell, -1 outside the hull, -2 walk did not terminate
+    int4 v;       // vertex ids of t
+    d3 X[4];      // vertex positions of t
+    double O[4];  // O_i(p)
+};
+
+__device__ __forceinline__ void face_orients(const d3 X[4], d3 p, double O[4], int cert[4]) {
+#pragma clang fp contract(off)
+    d3 d0 = sub(X[0], p), d1 = sub(X[1], p), d2 = sub(X[2], p), d3v = sub(X[3], p);
+    d3 x23 = cross(d2, d3v), x31 = cross(d3v, d1), x12 = cross(d1, d2);
+    O[0] = dot(d1, x23);
+    O[1] = -dot(d0, x23);
+    O[2] = -dot(d0, x31);
+    O[3] = -dot(d0, x12);
+    d3 p23 = abs_cross(d2, d3v), p31 = abs_cross(d3v, d1), p12 = abs_cross(d1, d2);
+    d3 a0 = dabs(d0), a1 = dabs(d1);
+    const double eb = 8.0e-16;  // (7 + 56 eps) eps, rounded up
+    double b0 = eb * dot(a1, p23), b1 = eb * dot(a0, p23), b2 = eb * dot(a0, p31), b3 = eb * dot(a0, p12);
+    cert[0] = O[0] > b0 ? 1 : (O[0] < -b0 ? -1 : 0);
+    cert[1] = O[1] > b1 ? 1 : (O[1] < -b1 ? -1 : 0);
+    cert[2] = O[2] > b2 ? 1 : (O[2] < -b2 ? -1 : 0);
+    cert[3] = O[3] > b3 ? 1 : (O[3] < -b3 ? -1 : 0);
+}
+
+// EXACT = false: signs the FP64 filter cannot certify end the walk with
+// t = -3 (the row is then redone by the exact kernel). EXACT = true: such
+// signs are decided exactly with the symbolic tie-break.
+template <bool EXACT>
+__device__ Located locate(const double *__restrict__ pos, const int4 *__restrict__ cells,
+                          const int4 *__restrict__ nbrs, d3 p, int t) {
+    Located L;
+    int prev = -1;
+    for (int step = 0; step < kMaxWalkSteps; ++step) {
+        int4 c = cells[t];
+        int4 nb4 = nbrs[t];
+        L.X[0] = load_pos(pos, c.x);
+        L.X[1] = load_pos(pos, c.y);
+        L.X[2] = load_pos(pos, c.z);
+        L.X[3] = load_pos(pos, c.w);
+        int cert[4];
+        face_orients(L.X, p, L.O, cert);
+        int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+        int pick = -1;
+        bool unsure = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (pick >= 0 || unsure) continue;
+            if (prev >= 0 && nb[i] == prev) continue;
+            int s = cert[i];
+            if (s == 0) {
+                if (EXACT) s = face_sos(i, L.X, p);
+                else unsure = true;
+            }
+            if (s < 0) pick = i;
+        }
+        if (unsure) {
+            L.t = -3;
+            return L;
+        }
+        if (pick < 0) {
+            L.t = t;
+            L.v = c;
+            return L;
+        }
+        int nn = pick == 0 ? nb[0] : pick == 1 ? nb[1] : pick == 2 ? nb[2] : nb[3];
+        if (nn < 0) {
+            L.t = -1;
+            return L;
+        }
+        prev = t;
+        t = nn;
+    }
+    L.t = -2;
+    return L;
+}
+
+// ----------------------------------------------------------------------------
+// per-tetrahedron linear fields (the reference's per-sample formulas)
+// ----------------------------------------------------------------------------
+// A = rows (x_{k+1} - x_0); inverse by cofactors; all zero when
+// |det| / sum|A_ij| <= 1e-6 (triangulation_miscellaneous.cc:77-95).
+__device__ __forceinline__ bool ref_inverse(const d3 X[4], double Ai[3][3]) {
+    double A[3][3];
+    A[0][0] = X[1].x - X[0].x; A[0][1] = X[1].y - X[0].y; A[0][2] = X[1].z - X[0].z;
+    A[1][0] = X[2].x - X[0].x; A[1][1] = X[2].y - X[0].y; A[1][2] = X[2].z - X[0].z;
+    A[2][0] = X[3].x - X[0].x; A[2][1] = X[3].y - X[0].y; A[2][2] = X[3].z - X[0].z;
+    double M[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            int i1 = (r + 1) % 3, i2 = (r + 2) % 3, j1 = (c + 1) % 3, j2 = (c + 2) % 3;
+            M[r][c] = A[j1][i1] * A[j2][i2] - A[j1][i2] * A[j2][i1];
+        }
+    double det = A[0][0] * M[0][0] + A[0][1] * M[1][0] + A[0][2] * M[2][0];
+    double s = fabs(A[0][0]) + fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][0]) + fabs(A[1][1]) +
+               fabs(A[1][2]) + fabs(A[2][0]) + fabs(A[2][1]) + fabs(A[2][2]);
+    bool ok = fabs(det / s) > 1.0e-6;
+    double inv = 1.0 / det;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Ai[r][c] = ok ? M[r][c] * inv : 0.0;
+    return ok;
+}
+
+template <typename F, int NS>
+struct RowAcc {
+    F d;
+    F v[3];
+    F vg[9];
+    F s[NS > 0 ? NS : 1];
+    F sg[3 * (NS > 0 ? NS : 1)];
+    F wmean[3];  // running velocity mean / M2 for the standard deviation
+    F wm2;
+    int wk;
+    __device__ void clear() {
+        d = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] = 0, wmean[i] = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) vg[i] = 0;
+#pragma unroll
+        for (int i = 0; i < (NS > 0 ? NS : 1); ++i) s[i] = 0;
+#pragma unroll
+        for (int i = 0; i < 3 * (NS > 0 ? NS : 1); ++i) sg[i] = 0;
+        wm2 = 0;
+        wk = 0;
+    }
+};
+
+struct VertexFieldsDev {
+    const void *rho;
+    const void *vel;
+    const void *scal;
+    int S;
+};
+
+template <typename F>
+__device__ __forceinline__ F ldf(const void *p, int64_t i) {
+    return reinterpret_cast<const F *>(p)[i];
+}
+
+// Adds the field values at p in the located cell to the row accumulator.
+// F = double: the reference's formulas evaluated in FP64.
+// F = float : FP64 inverse cast to FP32 (as the reference's float build does)
+//             and the density as the convex barycentric sum in FP32.
+template <typename F, bool VEL, int NS>
+__device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexFieldsDev &vf, unsigned fields,
+                                           bool value_bug, RowAcc<F, NS> &acc, bool want_std) {
+    double Ai[3][3];
+    bool ok = ref_inverse(L.X, Ai);
+    const int id[4] = {L.v.x, L.v.y, L.v.z, L.v.w};
+    double rel_d[3] = {p.x - L.X[0].x, p.y - L.X[0].y, p.z - L.X[0].z};
+    F rel[3] = {(F)rel_d[0], (F)rel_d[1], (F)rel_d[2]};
+    F AiF[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) AiF[r][c] = (F)Ai[r][c];
+    F lam[4];
+    if (sizeof(F) == 4) {
+        double osum = (L.O[0] + L.O[1]) + (L.O[2] + L.O[3]);
+        double inv = 1.0 / osum;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lam[i] = (F)(L.O[i] * inv);
+    }
+    if (fields & DTFE_FIELD_DENSITY) {
+        F r0 = ldf<F>(vf.rho, id[0]), r1 = ldf<F>(vf.rho, id[1]), r2 = ldf<F>(vf.rho, id[2]),
+          r3 = ldf<F>(vf.rho, id[3]);
+        F val;
+        if (sizeof(F) == 8) {
+            F dd[3] = {r1 - r0, r2 - r0, r3 - r0};
+            F g[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g[i] = AiF[i][0] * dd[0] + AiF[i][1] * dd[1] + AiF[i][2] * dd[2];
+            val = r0 + (rel[0] * g[0] + rel[1] * g[1] + rel[2] * g[2]);
+        } else {
+            val = ok ? (lam[0] * r0 + lam[1] * r1) + (lam[2] * r2 + lam[3] * r3) : r0;
+        }
+        acc.d += val;
+    }
+    if (VEL && (fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_STD))) {
+        F vb[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vb[k][c] = ldf<F>(vf.vel, 3 * (int64_t)id[k] + c);
+        F G[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                G[i][c] = AiF[i][0] * (vb[1][c] - vb[0][c]) + AiF[i][1] * (vb[2][c] - vb[0][c]) +
+                          AiF[i][2] * (vb[3][c] - vb[0][c]);
+        if (fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_STD)) {
+            F val[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (value_bug) {
+                    val[c] = G[2][c] * rel[2] + vb[0][c];
+                } else if (sizeof(F) == 8) {
+                    val[c] = (G[0][c] * rel[0] + G[1][c] * rel[1] + G[2][c] * rel[2]) + vb[0][c];
+                } else {
+                    val[c] = ok ? (lam[0] * vb[0][c] + lam[1] * vb[1][c]) + (lam[2] * vb[2][c] + lam[3] * vb[3][c])
+                                : vb[0][c];
+                }
+            }
+            if (fields & DTFE_FIELD_VELOCITY)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc.v[c] += val[c];
+            if (want_std) {
+                acc.wk += 1;
+                F inv = (F)1 / (F)acc.wk;
+                F m2 = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    F delta = val[c] - acc.wmean[c];
+                    acc.wmean[c] += delta * inv;
+                    m2 += delta * (val[c] - acc.wmean[c]);
+                }
+                acc.wm2 += m2;
+            }
+        }
+        if (fields & DTFE_FIELD_VELOCITY_GRADIENT)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc.vg[c * 3 + i] += G[i][c];
+    }
+    if (NS > 0 && (fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT))) {
+        const int S = vf.S;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            if (q >= S) break;
+            F sb[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sb[k] = ldf<F>(vf.scal, (int64_t)S * id[k] + q);
+            F g[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                g[i] = AiF[i][0] * (sb[1] - sb[0]) + AiF[i][1] * (sb[2] - sb[0]) + AiF[i][2] * (sb[3] - sb[0]);
+            if (fields & DTFE_FIELD_SCALAR) {
+                F val;
+                if (value_bug) val = g[2] * rel[2] + sb[0];
+                else if (sizeof(F) == 8) val = (g[0] * rel[0] + g[1] * rel[1] + g[2] * rel[2]) + sb[0];
+                else val = ok ? (lam[0] * sb[0] + lam[1] * sb[1]) + (lam[2] * sb[2] + lam[3] * sb[3]) : sb[0];
+                acc.s[q] += val;
+            }
+            if (fields & DTFE_FIELD_SCALAR_GRADIENT)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) acc.sg[q * 3 + i] += g[i];
+        }
+    }
+}
+
+// A sample outside the hull contributes a zero vector to the std.
+template <typename F, int NS>
+__device__ __forceinline__ void add_zero_std(RowAcc<F, NS> &acc) {
+    acc.wk += 1;
+    F inv = (F)1 / (F)acc.wk;
+    F m2 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        F delta = -acc.wmean[c];
+        acc.wmean[c] += delta * inv;
+        m2 += delta * (-acc.wmean[c]);
+    }
+    acc.wm2 += m2;
+}
+
+struct OutPtrs {
+    void *density, *velocity, *velgrad, *div, *shear, *vort, *velstd, *scalar, *scalargrad;
+    int32_t *cell;
+};
+
+template <typename F>
+__device__ __forceinline__ void stf(void *p, int64_t i, F v) {
+    reinterpret_cast<F *>(p)[i] = v;
+}
+
+// Writes one output row; div/shear/vort follow DTFE.cpp:394-433.
+template <typename F, bool VEL, int NS>
+__device__ __forceinline__ void store_row(const OutPtrs &o, int64_t row, const RowAcc<F, NS> &acc, F scale,
+                                          unsigned fields, int S) {
+    if (fields & DTFE_FIELD_DENSITY) stf<F>(o.density, row, acc.d * scale);
+    if (VEL) {
+        if (fields & DTFE_FIELD_VELOCITY)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) stf<F>(o.velocity, 3 * row + c, acc.v[c] * scale);
+        if (fields & (DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE | DTFE_FIELD_VELOCITY_SHEAR |
+                      DTFE_FIELD_VELOCITY_VORTICITY)) {
+            F g[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) g[i] = acc.vg[i] * scale;
+            if (fields & DTFE_FIELD_VELOCITY_GRADIENT)
+#pragma unroll
+                for (int i = 0; i < 9; ++i) stf<F>(o.velgrad, 9 * row + i, g[i]);
+            if (fields & DTFE_FIELD_VELOCITY_DIVERGENCE) stf<F>(o.div, row, g[0] + g[4] + g[8]);
+            if (fields & DTFE_FIELD_VELOCITY_SHEAR) {
+                F t0 = (g[0] + g[0]) / (F)2, t1 = (g[1] + g[3]) / (F)2, t2 = (g[2] + g[6]) / (F)2;
+                F t3 = (g[4] + g[4]) / (F)2, t4 = (g[5] + g[7]) / (F)2;
+                F tr = (t0 + t3 + g[8]) / (F)3;
+                stf<F>(o.shear, 5 * row + 0, t0 - tr);
+                stf<F>(o.shear, 5 * row + 1, t1);
+                stf<F>(o.shear, 5 * row + 2, t2);
+                stf<F>(o.shear, 5 * row + 3, t3 - tr);
+                stf<F>(o.shear, 5 * row + 4, t4);
+            }
+            if (fields & DTFE_FIELD_VELOCITY_VORTICITY) {
+                stf<F>(o.vort, 3 * row + 0, (g[1] - g[3]) / (F)2);
+                stf<F>(o.vort, 3 * row + 1, (g[2] - g[6]) / (F)2);
+                stf<F>(o.vort, 3 * row + 2, (g[5] - g[7]) / (F)2);
+            }
+        }
+    }
+    if (NS > 0) {
+        if (fields & DTFE_FIELD_SCALAR)
+            for (int q = 0; q < S; ++q) stf<F>(o.scalar, (int64_t)S * row + q, acc.s[q] * scale);
+        if (fields & DTFE_FIELD_SCALAR_GRADIENT)
+            for (int q = 0; q < 3 * S; ++q) stf<F>(o.scalargrad, (int64_t)3 * S * row + q, acc.sg[q] * scale);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// sampling kernel
+// ----------------------------------------------------------------------------
+struct SeedGrid {
+    const int *seed;
+    const double *bbox;  // device: lo[3], inv_h[3]
+    int g[3];
+};
+
+struct SampleArgs {
+    const double *pos;
+    const int4 *cells;
+    const int4 *nbrs;
+    int64_t nc;
+    SeedGrid sg;
+    VertexFieldsDev vf;
+    unsigned fields;
+    unsigned flags;
+    int kind, method, M, arg_order;
+    int64_t n_rows;
+    int64_t n[3];
+    const double *ax0, *ax1, *ax2;  // grid: centre coordinates; cone: r, theta, psi
+    const double *trig;             // cone: sin(theta)[n1], cos(theta)[n1], sin(psi)[n2], cos(psi)[n2]
+    double dx[3];
+    double origin[3];
+    const double *pts, *deltas;
+    const int64_t *row_point;       // user points kept after the region filter
+    const double *equi;             // [M][3] equidistant offsets
+    const uint32_t *plan;           // MT states, one per workgroup
+    int64_t rows_per_block;
+    int rows_per_round;
+    int ring;                       // MT blocks held in LDS
+    OutPtrs out;
+    int *status;                    // [0] walk failures, [1] deferred rows, [2] deferral overflow
+    int64_t *defer_rows;            // rows the exact kernel redoes
+    uint32_t *defer_words;          // their MC words, 3M per row
+    int defer_cap;
+};
+
+__device__ __forceinline__ int seed_cell(const SeedGrid &sg, d3 p) {
+    const double *b = sg.bbox;
+    int ix = (int)floor((p.x - b[0]) * b[3]);
+    int iy = (int)floor((p.y - b[1]) * b[4]);
+    int iz = (int)floor((p.z - b[2]) * b[5]);
+    ix = min(max(ix, 0), sg.g[0] - 1);
+    iy = min(max(iy, 0), sg.g[1] - 1);
+    iz = min(max(iz, 0), sg.g[2] - 1);
+    int s = sg.seed[((int64_t)ix * sg.g[1] + iy) * sg.g[2] + iz];
+    return s < 0 ? 0 : s;
+}
+
+__device__ __forceinline__ d3 row_centre(const SampleArgs &a, int64_t row) {
+    if (a.kind == DTFE_SAMPLE_GRID) {
+        int64_t k = row % a.n[2];
+        int64_t j = (row / a.n[2]) % a.n[1];
+        int64_t i = row / (a.n[1] * a.n[2]);
+        return d3{a.ax0[i], a.ax1[j], a.ax2[k]};
+    } else if (a.kind == DTFE_SAMPLE_CONE) {
+#pragma clang fp contract(off)
+        int64_t k = row % a.n[2];
+        int64_t j = (row / a.n[2]) % a.n[1];
+        int64_t i = row / (a.n[1] * a.n[2]);
+        double r = a.ax0[i];
+        double st = a.trig[j], ct = a.trig[a.n[1] + j];
+        double sp = a.trig[2 * a.n[1] + k], cp = a.trig[2 * a.n[1] + a.n[2] + k];
+        return d3{a.origin[0] + r * st * cp, a.origin[1] + r * st * sp, a.origin[2] + r * ct};
+    } else {
+        const double *q = a.pts + 3 * a.row_point[row];
+        return d3{q[0], q[1], q[2]};
+    }
+}
+
+__device__ __forceinline__ d3 row_coords(const SampleArgs &a, int64_t row) {
+    // the unconverted cell coordinates (grid x,y,z or cone r,theta,psi)
+    int64_t k = row % a.n[2];
+    int64_t j = (row / a.n[2]) % a.n[1];
+    int64_t i = row / (a.n[1] * a.n[2]);
+    return d3{a.ax0[i], a.ax1[j], a.ax2[k]};
+}
+
+// MC sample point from three uniforms u[0..2] (in stream order).
+__device__ __forceinline__ d3 mc_point(const SampleArgs &a, int64_t row, d3 centre, const double u[3]) {
+    if (a.kind == DTFE_SAMPLE_GRID) {
+        double ux = a.arg_order == 0 ? u[0] : u[2];
+        double uz = a.arg_order == 0 ? u[2] : u[0];
+        return d3{mul_add_rn(ux, a.dx[0], centre.x), mul_add_rn(u[1], a.dx[1], centre.y),
+                  mul_add_rn(uz, a.dx[2], centre.z)};
+    } else if (a.kind == DTFE_SAMPLE_CONE) {
+#pragma clang fp contract(off)
+        d3 c = row_coords(a, row);
+        double tr = c.x + u[0] * a.dx[0];
+        double tt = c.y + u[1] * a.dx[1];
+        double tp = c.z + u[2] * a.dx[2];
+        double st = sin(tt), ct = cos(tt), sp = sin(tp), cp = cos(tp);
+        return d3{a.origin[0] + tr * st * cp, a.origin[1] + tr * st * sp, a.origin[2] + tr * ct};
+    } else {
+        const double *dl = a.deltas + 3 * a.row_point[row];
+        return d3{add_mul_rn(centre.x, u[0], dl[0]), add_mul_rn(centre.y, u[1], dl[1]),
+                  add_mul_rn(centre.z, u[2], dl[2])};
+    }
+}
+
+// Source of the row's Monte-Carlo words: the LDS ring of the MC kernel or
+// the words saved with a deferred row.
+struct RingWords {
+    const uint32_t *ring;
+    int64_t base;  // MT block held in slot 0
+    int n;         // slots
+    __device__ __forceinline__ uint32_t operator()(int64_t word) const {
+        int64_t blk = word / MT_N;
+        int slot = (int)(blk - base) % n;
+        return ring[slot * MT_N + (int)(word - blk * MT_N)];
+    }
+};
+
+struct SavedWords {
+    const uint32_t *w;  // this row's 3M words
+    int64_t first;      // stream index of w[0]
+    __device__ __forceinline__ uint32_t operator()(int64_t word) const { return w[word - first]; }
+};
+
+// Returns false when a sign could not be certified (EXACT = false only):
+// nothing is written and the caller defers the row.
+template <typename F, bool VEL, int NS, bool EXACT, typename Words>
+__device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, const Words &words) {
+    const bool value_bug = (a.flags & DTFE_COMPAT_VALUE_BUG) != 0;
+    d3 centre = row_centre(a, row);
+    int start = seed_cell(a.sg, centre);
+    Located Lc = locate<EXACT>(a.pos, a.cells, a.nbrs, centre, start);
+    if (!EXACT && Lc.t == -3) return false;
+    if (Lc.t == -2) {
+        atomicAdd(a.status, 1);
+        Lc.t = -1;
+    }
+    RowAcc<F, NS> acc;
+    acc.clear();
+    if (a.method == DTFE_METHOD_CENTRE) {
+        if (a.out.cell) a.out.cell[row] = Lc.t;
+        if (Lc.t >= 0) add_sample<F, VEL, NS>(Lc, centre, a.vf, a.fields, value_bug, acc, false);
+        store_row<F, VEL, NS>(a.out, row, acc, (F)1, a.fields, a.vf.S);
+        return true;
+    }
+    const bool want_std = VEL && (a.fields & DTFE_FIELD_VELOCITY_STD) && a.kind == DTFE_SAMPLE_GRID &&
+                          a.method == DTFE_METHOD_MC;
+    int hint = Lc.t >= 0 ? Lc.t : start;
+    int finite = 0;
+    for (int w = 0; w < a.M; ++w) {
+        d3 p;
+        if (a.method == DTFE_METHOD_EQUI) {
+            const double *e = a.equi + 3 * w;
+            p = d3{add_rn(centre.x, e[0]), add_rn(centre.y, e[1]), add_rn(centre.z, e[2])};
+        } else {
+            int64_t word = 3 * ((int64_t)a.M * row + w);
+            double u[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) u[q] = mt_uniform(mt_temper(words(word + q)));
+            p = mc_point(a, row, centre, u);
+        }
+        Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, p, hint);
+        if (!EXACT && L.t == -3) return false;
+        if (L.t == -2) {
+            atomicAdd(a.status, 1);
+            L.t = -1;
+        }
+        if (L.t < 0) {
+            if (want_std) add_zero_std<F, NS>(acc);
+            continue;
+        }
+        ++finite;
+        add_sample<F, VEL, NS>(L, p, a.vf, a.fields, value_bug, acc, want_std);
+    }
+    if (a.out.cell) a.out.cell[row] = Lc.t;
+    F scale;
+    if (a.flags & DTFE_AVERAGE_FINITE_ONLY) scale = finite > 0 ? (F)1 / (F)finite : (F)0;
+    else scale = (F)1 / (F)a.M;
+    store_row<F, VEL, NS>(a.out, row, acc, scale, a.fields, a.vf.S);
+    if (want_std) {
+        F var = acc.wm2 / (F)a.M;
+        stf<F>(a.out.velstd, row, sqrt(var));
+    }
+    return true;
+}
+
+// Queues a row for the exact kernel, with its Monte-Carlo words.
+template <typename Words>
+__device__ __forceinline__ void defer_row(const SampleArgs &a, int64_t row, const Words &words) {
+    int slot = atomicAdd(a.status + 1, 1);
+    if (slot >= a.defer_cap) {
+        atomicOr(a.status + 2, 1);
+        return;
+    }
+    a.defer_rows[slot] = row;
+    if (a.method == DTFE_METHOD_MC) {
+        const int64_t wpr = 3 * (int64_t)a.M;
+        for (int64_t q = 0; q < wpr; ++q) a.defer_words[(int64_t)slot * wpr + q] = words(wpr * row + q);
+    }
+}
+
+struct NoWords {
+    __device__ __forceinline__ uint32_t operator()(int64_t) const { return 0u; }
+};
+
+// Generates MT block q into its ring slot from block q - 1 (the previous
+// slot). new[i] depends on old[i], old[i+1] and old[i+397] for i < 227 and
+// on new[i-227] otherwise; old[624] is new[0]. Three phases, one barrier
+// each; the leading barrier protects readers of the slot being overwritten.
+__device__ __forceinline__ void mt_generate_block(uint32_t *ring, int64_t q, int64_t base, int n) {
+    __syncthreads();
+    const uint32_t *old = ring + (int)((q - 1 - base) % n) * MT_N;
+    uint32_t *nw = ring + (int)((q - base) % n) * MT_N;
+    const int t = threadIdx.x;
+    if (t < 227) {
+        uint32_t y = (old[t] & MT_UPPER) | (old[t + 1] & MT_LOWER);
+        uint32_t v = old[t + MT_M] ^ (y >> 1);
+        nw[t] = (y & 1u) ? (v ^ MT_MATRIX) : v;
+    }
+    __syncthreads();
+    if (t < 227) {
+        int i = t + 227;
+        uint32_t y = (old[i] & MT_UPPER) | (old[i + 1] & MT_LOWER);
+        uint32_t v = nw[i - 227] ^ (y >> 1);
+        nw[i] = (y & 1u) ? (v ^ MT_MATRIX) : v;
+    }
+    __syncthreads();
+    if (t < MT_N - 454) {
+        int i = t + 454;
+        uint32_t nxt = (i == MT_N - 1) ? nw[0] : old[i + 1];
+        uint32_t y = (old[i] & MT_UPPER) | (nxt & MT_LOWER);
+        uint32_t v = nw[i - 227] ^ (y >> 1);
+        nw[i] = (y & 1u) ? (v ^ MT_MATRIX) : v;
+    }
+}
+
+// Rows without random draws: one thread per row.
+template <typename F, bool VEL, int NS>
+__global__ void __launch_bounds__(kBlock) k_sample_plain(SampleArgs a) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    NoWords nw;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride)
+        if (!process_row<F, VEL, NS, false>(a, row, nw)) defer_row(a, row, nw);
+}
+
+// Monte-Carlo rows: workgroup b owns rows [b*rows_per_block, ...) and
+// regenerates their MT19937 words in an LDS ring of `ring` MT blocks,
+// starting from the plan state of its first block.
+template <typename F, bool VEL, int NS>
+__global__ void __launch_bounds__(kBlock) k_sample_mc(SampleArgs a) {
+    extern __shared__ uint32_t ring[];
+    const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
+    if (row0 >= a.n_rows) return;
+    const int64_t row_end = min(row0 + a.rows_per_block, a.n_rows);
+    const int64_t wpr = 3 * (int64_t)a.M;
+    const int64_t first_block = (wpr * row0) / MT_N;
+    // slot 0 holds the plan state, i.e. MT block first_block - 1
+    const int64_t base = first_block - 1;
+    for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
+    RingWords words{ring, base, a.ring};
+    int64_t have = first_block - 1;  // last generated block
+    for (int64_t r0 = row0; r0 < row_end; r0 += a.rows_per_round) {
+        int64_t r1 = min(r0 + a.rows_per_round, row_end);
+        int64_t need = (wpr * r1 - 1) / MT_N;
+        while (have < need) {
+            mt_generate_block(ring, have + 1, base, a.ring);
+            ++have;
+        }
+        __syncthreads();
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
+            if (!process_row<F, VEL, NS, false>(a, row, words)) defer_row(a, row, words);
+    }
+}
+
+// Deferred rows, redone with exact signs (symbolic tie-break).
+template <typename F, bool VEL, int NS>
+__global__ void __launch_bounds__(kBlock) k_sample_exact(SampleArgs a, int n_deferred) {
+    int stride = gridDim.x * blockDim.x;
+    const int64_t wpr = 3 * (int64_t)a.M;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_deferred; i += stride) {
+        int64_t row = a.defer_rows[i];
+        SavedWords words{a.defer_words + (int64_t)i * wpr, wpr * row};
+        process_row<F, VEL, NS, true>(a, row, words);
+    }
+}
+
+// Every row with exact signs (used when the deferred queue overflowed).
+template <typename F, bool VEL, int NS>
+__global__ void __launch_bounds__(kBlock) k_sample_exact_all(SampleArgs a) {
+    extern __shared__ uint32_t ring[];
+    if (a.method != DTFE_METHOD_MC) {
+        int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        NoWords nw;
+        for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride)
+            process_row<F, VEL, NS, true>(a, row, nw);
+        return;
+    }
+    const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
+    if (row0 >= a.n_rows) return;
+    const int64_t row_end = min(row0 + a.rows_per_block, a.n_rows);
+    const int64_t wpr = 3 * (int64_t)a.M;
+    const int64_t first_block = (wpr * row0) / MT_N;
+    const int64_t base = first_block - 1;
+    for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
+    RingWords words{ring, base, a.ring};
+    int64_t have = first_block - 1;
+    for (int64_t r0 = row0; r0 < row_end; r0 += a.rows_per_round) {
+        int64_t r1 = min(r0 + a.rows_per_round, row_end);
+        int64_t need = (wpr * r1 - 1) / MT_N;
+        while (have < need) {
+            mt_generate_block(ring, have + 1, base, a.ring);
+            ++have;
+        }
+        __syncthreads();
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
+            process_row<F, VEL, NS, true>(a, row, words);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// vertex density
+// ----------------------------------------------------------------------------
+__global__ void k_cell_volumes(const double *__restrict__ pos, const int4 *__restrict__ cells,
+                               const int4 *__restrict__ nbrs, int64_t nc, double *__restrict__ W,
+                               uint8_t *__restrict__ hull) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
+        int4 c = cells[t];
+        d3 p0 = load_pos(pos, c.x), p1 = load_pos(pos, c.y), p2 = load_pos(pos, c.z), p3 = load_pos(pos, c.w);
+        d3 a = sub(p1, p0), b = sub(p2, p0), d = sub(p3, p0);
+        double det = a.x * (b.y * d.z - b.z * d.y) - a.y * (b.x * d.z - b.z * d.x) + a.z * (b.x * d.y - b.y * d.x);
+        double vol = det / 6.0;
+        atomicAdd(W + c.x, vol);
+        atomicAdd(W + c.y, vol);
+        atomicAdd(W + c.z, vol);
+        atomicAdd(W + c.w, vol);
+        int4 nb = nbrs[t];
+        // a face on the hull: its three vertices touch an infinite cell
+        if (nb.x < 0) { hull[c.y] = 1; hull[c.z] = 1; hull[c.w] = 1; }
+        if (nb.y < 0) { hull[c.x] = 1; hull[c.z] = 1; hull[c.w] = 1; }
+        if (nb.z < 0) { hull[c.x] = 1; hull[c.y] = 1; hull[c.w] = 1; }
+        if (nb.w < 0) { hull[c.x] = 1; hull[c.y] = 1; hull[c.z] = 1; }
+    }
+}
+
+template <typename F>
+__global__ void k_vertex_density(const F *__restrict__ w, const double *__restrict__ W,
+                                 const uint8_t *__restrict__ hull, int64_t nv, double factor, F *__restrict__ rho) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
+        rho[v] = hull[v] ? (F)0 : (F)((double)w[v] * factor / W[v]);
+}
+
+// ----------------------------------------------------------------------------
+// seed grid for starting walks
+// ----------------------------------------------------------------------------
+__global__ void k_bbox_partial(const double *__restrict__ pos, int64_t nv, double *__restrict__ part) {
+    __shared__ double s[6][kBlock];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
+        for (int k = 0; k < 3; ++k) {
+            double x = pos[3 * v + k];
+            lo[k] = fmin(lo[k], x);
+            hi[k] = fmax(hi[k], x);
+        }
+    for (int k = 0; k < 3; ++k) {
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 3; ++k) {
+                s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + off]);
+                s[3 + k][threadIdx.x] = fmax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + off]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = s[k][0];
+}
+
+// bbox[0..2] = lo, bbox[3..5] = g / (hi - lo)
+__global__ void k_bbox_final(const double *__restrict__ part, int nparts, int gx, int gy, int gz,
+                             double *__restrict__ bbox) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int b = 0; b < nparts; ++b)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fmin(lo[k], part[6 * b + k]);
+            hi[k] = fmax(hi[k], part[6 * b + 3 + k]);
+        }
+    int g[3] = {gx, gy, gz};
+    for (int k = 0; k < 3; ++k) {
+        double ext = hi[k] - lo[k];
+        bbox[k] = lo[k];
+        bbox[3 + k] = ext > 0 ? g[k] / ext : 0.0;
+    }
+}
+
+__global__ void k_seed_cells(const double *__restrict__ pos, const int4 *__restrict__ cells, int64_t nc,
+                             SeedGrid sg, int *__restrict__ seed) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
+        int4 c = cells[t];
+        d3 p0 = load_pos(pos, c.x), p1 = load_pos(pos, c.y), p2 = load_pos(pos, c.z), p3 = load_pos(pos, c.w);
+        d3 m{0.25 * (p0.x + p1.x + p2.x + p3.x), 0.25 * (p0.y + p1.y + p2.y + p3.y),
+             0.25 * (p0.z + p1.z + p2.z + p3.z)};
+        const double *b = sg.bbox;
+        int ix = min(max((int)floor((m.x - b[0]) * b[3]), 0), sg.g[0] - 1);
+        int iy = min(max((int)floor((m.y - b[1]) * b[4]), 0), sg.g[1] - 1);
+        int iz = min(max((int)floor((m.z - b[2]) * b[5]), 0), sg.g[2] - 1);
+        seed[((int64_t)ix * sg.g[1] + iy) * sg.g[2] + iz] = (int)t;
+    }
+}
+
+// empty buckets copy a neighbouring bucket's seed
+__global__ void k_seed_fill(const int *__restrict__ in, int *__restrict__ out, int gx, int gy, int gz) {
+    int64_t n = (int64_t)gx * gy * gz;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += stride) {
+        int s = in[b];
+        if (s < 0) {
+            int iz = (int)(b % gz), iy = (int)((b / gz) % gy), ix = (int)(b / ((int64_t)gy * gz));
+            const int dxs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+            for (int q = 0; q < 6 && s < 0; ++q) {
+                int jx = ix + dxs[q][0], jy = iy + dxs[q][1], jz = iz + dxs[q][2];
+                if (jx < 0 || jy < 0 || jz < 0 || jx >= gx || jy >= gy || jz >= gz) continue;
+                s = in[((int64_t)jx * gy + jy) * gz + jz];
+            }
+        }
+        out[b] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// user-point region filter (points outside `box` produce no row)
+// ----------------------------------------------------------------------------
+__global__ void k_points_in_box(const double *__restrict__ pts, int64_t n, const double *__restrict__ box,
+                                int *__restrict__ flag) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double *q = pts + 3 * i;
+        flag[i] = (q[0] >= box[0] && q[0] <= box[1] && q[1] >= box[2] && q[1] <= box[3] && q[2] >= box[4] &&
+                   q[2] <= box[5]) ? 1 : 0;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// CIC / TSC deposits
+// ----------------------------------------------------------------------------
+struct DepositArgs {
+    const double *pos;
+    const void *w;
+    const void *vel;
+    int64_t np;
+    double box[6];
+    double inner[6], outer[6];
+    double dx[3];
+    int n[3];
+    int scheme;
+    void *mass;  // [ncell]
+    void *mom;   // [ncell][3]
+    int want_vel;
+};
+
+template <typename F>
+__global__ void __launch_bounds__(kBlock) k_deposit(DepositArgs a) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    F *mass = reinterpret_cast<F *>(a.mass);
+    F *mom = reinterpret_cast<F *>(a.mom);
+    const F *wv = reinterpret_cast<const F *>(a.w);
+    const F *vv = reinterpret_cast<const F *>(a.vel);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.np; i += stride) {
+        const double *q = a.pos + 3 * i;
+        const double pc[3] = {q[0], q[1], q[2]};
+        bool in = pc[0] >= a.inner[0] && pc[0] <= a.inner[1] && pc[1] >= a.inner[2] && pc[1] <= a.inner[3] &&
+                  pc[2] >= a.inner[4] && pc[2] <= a.inner[5];
+        bool out = !in && pc[0] >= a.outer[0] && pc[0] <= a.outer[1] && pc[1] >= a.outer[2] &&
+                   pc[1] <= a.outer[3] && pc[2] >= a.outer[4] && pc[2] <= a.outer[5];
+        if (!in && !out) continue;
+        int cell[3][3], cnt[3];
+        F wt[3][3];
+        for (int j = 0; j < 3; ++j) {
+            F t = (F)((pc[j] - a.box[2 * j]) / a.dx[j]);
+            int ti = (int)floor(t);
+            int ng = a.n[j];
+            t -= (F)ti + (F)0.5;
+            const F h = (F)0.5;
+            F wl = a.scheme == 0 ? -t : h * (h - t) * (h - t);
+            F wc = a.scheme == 0 ? (F)1 + t : (F)0.75 - t * t;
+            F wr = a.scheme == 0 ? t : h * (h + t) * (h + t);
+            if (a.scheme == 0) {
+                // CIC: weights by the sign of t (CIC_interpolation.cc:127-138)
+                if (t < (F)0) { wl = -t; wc = (F)1 + t; wr = (F)0; }
+                else { wl = (F)0; wc = (F)1 - t; wr = t; }
+            }
+            if (in) {
+                cell[j][0] = ti - 1; cell[j][1] = ti; cell[j][2] = ti + 1; cnt[j] = 3;
+                wt[j][0] = wl; wt[j][1] = wc; wt[j][2] = wr;
+            } else {
+                cnt[j] = 0;
+                if (ti == -1) {
+                    cell[j][0] = 0; cnt[j] = 1;
+                    wt[j][0] = a.scheme == 0 ? t : h * (h + t) * (h + t);
+                } else if (ti == 0) {
+                    cell[j][0] = 0; cell[j][1] = 1; cnt[j] = 2;
+                    if (a.scheme == 0) { wt[j][0] = (F)1 - t; wt[j][1] = t; }
+                    else { wt[j][0] = (F)0.75 - t * t; wt[j][1] = h * (h + t) * (h + t); }
+                } else if (ti == ng - 1) {
+                    cell[j][0] = ng - 2; cell[j][1] = ng - 1; cnt[j] = 2;
+                    if (a.scheme == 0) { wt[j][0] = -t; wt[j][1] = (F)1 + t; }
+                    else { wt[j][0] = h * (h - t) * (h - t); wt[j][1] = (F)0.75 - t * t; }
+                } else if (ti == ng) {
+                    cell[j][0] = ng - 1; cnt[j] = 1;
+                    wt[j][0] = a.scheme == 0 ? -t : h * (h - t) * (h - t);
+                } else if (!(ti < -1 || ti > ng)) {
+                    cell[j][0] = ti - 1; cell[j][1] = ti; cell[j][2] = ti + 1; cnt[j] = 3;
+                    wt[j][0] = wl; wt[j][1] = wc; wt[j][2] = wr;
+                }
+            }
+        }
+        F wi = wv[i];
+        F v3[3] = {0, 0, 0};
+        if (a.want_vel)
+            for (int c = 0; c < 3; ++c) v3[c] = vv[3 * i + c];
+        for (int i1 = 0; i1 < cnt[0]; ++i1)
+            for (int i2 = 0; i2 < cnt[1]; ++i2)
+                for (int i3 = 0; i3 < cnt[2]; ++i3) {
+                    int64_t idx = ((int64_t)cell[0][i1] * a.n[1] + cell[1][i2]) * a.n[2] + cell[2][i3];
+                    F r = wi * wt[0][i1] * wt[1][i2] * wt[2][i3];
+                    atomicAdd(mass + idx, r);
+                    if (a.want_vel)
+                        for (int c = 0; c < 3; ++c) atomicAdd(mom + 3 * idx + c, v3[c] * r);
+                }
+    }
+}
+
+template <typename F>
+__global__ void k_deposit_finish(const F *__restrict__ mass, const F *__restrict__ mom, int64_t ncell,
+                                 F factor, F *__restrict__ density, F *__restrict__ velocity) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += stride) {
+        F m = mass[i];
+        if (velocity)
+            for (int c = 0; c < 3; ++c) velocity[3 * i + c] = m != (F)0 ? mom[3 * i + c] / m : (F)0;
+        if (density) density[i] = m * factor;
+    }
+}
+
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace dtfe
+
+// ============================================================================
+// host side
+// ============================================================================
+using namespace dtfe;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct dtfe_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf W, hull, seed_a, seed_b, bbox_part, bbox, status, axes, trig, equi, plan, flags, rows, mass, mom,
+        defer_rows, defer_words;
+    int last_deferred = 0;
+    // MC plan cache
+    bool plan_valid = false;
+    uint32_t plan_seed = 0;
+    int64_t plan_wpr = 0, plan_rows = 0, plan_rpb = 0;
+    std::vector<uint32_t> plan_host;
+    double last_plan_ms = 0.0;
+};
+
+namespace {
+
+int fail(dtfe_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, DTFE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(dtfe_ctx *ctx, DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return DTFE_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        (void)e;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t sz = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&b.p, sz) != hipSuccess) {
+        b.p = nullptr;
+        return fail(ctx, DTFE_ERR_NO_MEMORY, "device allocation of " + std::to_string(sz) + " bytes failed");
+    }
+    b.bytes = sz;
+    return DTFE_OK;
+}
+
+int grid_for(int64_t n, int block = kBlock) {
+    int64_t g = (n + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
+}
+
+int check_mesh(dtfe_ctx *ctx, const dtfe_mesh *m) {
+    if (!m) return fail(ctx, DTFE_ERR_ARGUMENT, "mesh is NULL");
+    if (m->n_vertices < 0 || m->n_cells < 0) return fail(ctx, DTFE_ERR_ARGUMENT, "negative mesh size");
+    if (m->n_vertices > INT32_MAX || m->n_cells > INT32_MAX)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "mesh larger than 2^31 vertices or cells");
+    if (m->n_vertices >= 4 && m->n_cells > 0 && (!m->pos || !m->cells || !m->neighbors))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "mesh arrays are NULL");
+    return DTFE_OK;
+}
+
+// Builds the seed grid of the mesh; returns it in `sg`.
+int build_seed_grid(dtfe_ctx *ctx, const dtfe_mesh *m, SeedGrid &sg) {
+    int64_t nv = m->n_vertices, nc = m->n_cells;
+    int g = (int)std::max<double>(1.0, std::floor(std::cbrt(std::max<double>(1.0, nv / 2.0))));
+    g = std::min(g, 512);
+    int64_t nb = (int64_t)g * g * g;
+    int parts = std::min(1024, grid_for(nv));
+    int rc;
+    if ((rc = ensure(ctx, ctx->bbox_part, 6 * sizeof(double) * parts))) return rc;
+    if ((rc = ensure(ctx, ctx->bbox, 6 * sizeof(double)))) return rc;
+    if ((rc = ensure(ctx, ctx->seed_a, nb * sizeof(int)))) return rc;
+    if ((rc = ensure(ctx, ctx->seed_b, nb * sizeof(int)))) return rc;
+    hipLaunchKernelGGL(k_bbox_partial, dim3(parts), dim3(kBlock), 0, ctx->stream, m->pos, nv,
+                       (double *)ctx->bbox_part.p);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)ctx->bbox_part.p, parts, g,
+                       g, g, (double *)ctx->bbox.p);
+    hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, (int *)ctx->seed_a.p, nb, -1);
+    sg.seed = (const int *)ctx->seed_a.p;
+    sg.bbox = (const double *)ctx->bbox.p;
+    sg.g[0] = sg.g[1] = sg.g[2] = g;
+    hipLaunchKernelGGL(k_seed_cells, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+                       (const int4 *)m->cells, nc, sg, (int *)ctx->seed_a.p);
+    int *in = (int *)ctx->seed_a.p, *out = (int *)ctx->seed_b.p;
+    for (int it = 0; it < 8; ++it) {
+        hipLaunchKernelGGL(k_seed_fill, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, in, out, g, g, g);
+        std::swap(in, out);
+    }
+    sg.seed = in;
+    HIP_TRY(hipGetLastError());
+    return DTFE_OK;
+}
+
+// which: 0 = fast kernel, 1 = deferred rows (exact), 2 = every row exact
+template <typename F, bool VEL, int NS>
+void launch_sample(const SampleArgs &a, int which, int mc, int grid, size_t lds, int n_def, hipStream_t s) {
+    if (which == 1)
+        hipLaunchKernelGGL((k_sample_exact<F, VEL, NS>), dim3(grid), dim3(kBlock), 0, s, a, n_def);
+    else if (which == 2)
+        hipLaunchKernelGGL((k_sample_exact_all<F, VEL, NS>), dim3(grid), dim3(kBlock), mc ? lds : 0, s, a);
+    else if (mc)
+        hipLaunchKernelGGL((k_sample_mc<F, VEL, NS>), dim3(grid), dim3(kBlock), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_sample_plain<F, VEL, NS>), dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+template <typename F, bool VEL>
+void launch_ns(const SampleArgs &a, int ns, int which, int mc, int grid, size_t lds, int n_def, hipStream_t s) {
+    switch (ns) {
+        case 0: launch_sample<F, VEL, 0>(a, which, mc, grid, lds, n_def, s); break;
+        case 1: launch_sample<F, VEL, 1>(a, which, mc, grid, lds, n_def, s); break;
+        case 2: launch_sample<F, VEL, 2>(a, which, mc, grid, lds, n_def, s); break;
+        case 3: launch_sample<F, VEL, 3>(a, which, mc, grid, lds, n_def, s); break;
+        default: launch_sample<F, VEL, 4>(a, which, mc, grid, lds, n_def, s); break;
+    }
+}
+
+void launch_any(const SampleArgs &a, bool f64, bool vel, int ns, int which, int mc, int grid, size_t lds, int n_def,
+                hipStream_t s) {
+    if (f64) {
+        if (vel) launch_ns<double, true>(a, ns, which, mc, grid, lds, n_def, s);
+        else launch_ns<double, false>(a, ns, which, mc, grid, lds, n_def, s);
+    } else {
+        if (vel) launch_ns<float, true>(a, ns, which, mc, grid, lds, n_def, s);
+        else launch_ns<float, false>(a, ns, which, mc, grid, lds, n_def, s);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dtfe_hip_version(void) { return "dtfe_hip 0.1 (gfx950)"; }
+
+int dtfe_hip_create(int device, dtfe_ctx **out) {
+    if (!out) return DTFE_ERR_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DTFE_ERR_HIP;
+    if (device < 0 || device >= n) return DTFE_ERR_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return DTFE_ERR_HIP;
+    dtfe_ctx *ctx = new dtfe_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return DTFE_ERR_HIP;
+    }
+    ctx->stream = ctx->own;
+    *out = ctx;
+    return DTFE_OK;
+}
+
+int dtfe_hip_destroy(dtfe_ctx *ctx) {
+    if (!ctx) return DTFE_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_a, &ctx->seed_b, &ctx->bbox_part, &ctx->bbox,
+                      &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
+                      &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return DTFE_OK;
+}
+
+int dtfe_hip_set_stream(dtfe_ctx *ctx, void *s) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    ctx->stream = (hipStream_t)s;  // NULL: the device's null stream
+    return DTFE_OK;
+}
+
+int dtfe_hip_synchronize(dtfe_ctx *ctx) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DTFE_OK;
+}
+
+const char *dtfe_hip_last_error(dtfe_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *m, int dtype, const void *weight, double rho_bar,
+                            void *rho_out) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    int rc = check_mesh(ctx, m);
+    if (rc) return rc;
+    if (dtype != DTFE_F32 && dtype != DTFE_F64) return fail(ctx, DTFE_ERR_ARGUMENT, "dtype must be F32 or F64");
+    if (!(rho_bar > 0.0)) return fail(ctx, DTFE_ERR_ARGUMENT, "average density must be positive");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int64_t nv = m->n_vertices, nc = m->n_cells;
+    size_t esz = dtype == DTFE_F64 ? 8 : 4;
+    if (nv < 4) {
+        if (nv > 0 && rho_out) {
+            if (dtype == DTFE_F64)
+                hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, ctx->stream, (double *)rho_out, nv, 0.0);
+            else
+                hipLaunchKernelGGL(k_fill<float>, dim3(1), dim3(64), 0, ctx->stream, (float *)rho_out, nv, 0.0f);
+        }
+        return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: no tetrahedra, densities are 0");
+    }
+    if (!weight || !rho_out) return fail(ctx, DTFE_ERR_ARGUMENT, "weight or output is NULL");
+    if ((rc = ensure(ctx, ctx->W, nv * sizeof(double)))) return rc;
+    if ((rc = ensure(ctx, ctx->hull, nv))) return rc;
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->W.p, nv, 0.0);
+    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (uint8_t *)ctx->hull.p, nv,
+                       (uint8_t)0);
+    hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+                       (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
+                       (uint8_t *)ctx->hull.p);
+    double factor = (3 + 1.0) / rho_bar;
+    if (dtype == DTFE_F64)
+        hipLaunchKernelGGL(k_vertex_density<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                           (const double *)weight, (const double *)ctx->W.p, (const uint8_t *)ctx->hull.p, nv, factor,
+                           (double *)rho_out);
+    else
+        hipLaunchKernelGGL(k_vertex_density<float>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                           (const float *)weight, (const double *)ctx->W.p, (const uint8_t *)ctx->hull.p, nv, factor,
+                           (float *)rho_out);
+    HIP_TRY(hipGetLastError());
+    return DTFE_OK;
+}
+
+int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *vf, unsigned fields,
+                    const dtfe_sampling *smp, const dtfe_mc *mc, unsigned flags, dtfe_outputs *out) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    int rc = check_mesh(ctx, m);
+    if (rc) return rc;
+    if (!vf || !smp || !mc || !out) return fail(ctx, DTFE_ERR_ARGUMENT, "NULL argument");
+    if (vf->dtype != DTFE_F32 && vf->dtype != DTFE_F64) return fail(ctx, DTFE_ERR_ARGUMENT, "bad dtype");
+    if (vf->n_scalar < 0 || vf->n_scalar > kMaxScalars)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "n_scalar must be in [0, 4]");
+    const bool want_vel = fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE |
+                                    DTFE_FIELD_VELOCITY_SHEAR | DTFE_FIELD_VELOCITY_VORTICITY | DTFE_FIELD_VELOCITY_STD);
+    const bool want_scal = fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT);
+    if ((fields & DTFE_FIELD_DENSITY) && (!vf->density || !out->density))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "density requested without vertex densities or output");
+    if (want_vel && !vf->velocity) return fail(ctx, DTFE_ERR_ARGUMENT, "velocity fields requested without velocities");
+    if (want_scal && (!vf->scalar || vf->n_scalar == 0))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "scalar fields requested without scalars");
+    struct { unsigned bit; void *p; const char *name; } req[] = {
+        {DTFE_FIELD_VELOCITY, out->velocity, "velocity"},
+        {DTFE_FIELD_VELOCITY_GRADIENT, out->velocity_gradient, "velocity_gradient"},
+        {DTFE_FIELD_VELOCITY_DIVERGENCE, out->velocity_divergence, "velocity_divergence"},
+        {DTFE_FIELD_VELOCITY_SHEAR, out->velocity_shear, "velocity_shear"},
+        {DTFE_FIELD_VELOCITY_VORTICITY, out->velocity_vorticity, "velocity_vorticity"},
+        {DTFE_FIELD_VELOCITY_STD, out->velocity_std, "velocity_std"},
+        {DTFE_FIELD_SCALAR, out->scalar, "scalar"},
+        {DTFE_FIELD_SCALAR_GRADIENT, out->scalar_gradient, "scalar_gradient"}};
+    for (auto &r : req)
+        if ((fields & r.bit) && !r.p) return fail(ctx, DTFE_ERR_ARGUMENT, std::string("output for ") + r.name + " is NULL");
+    const int method = mc->method;
+    if (method != DTFE_METHOD_CENTRE && method != DTFE_METHOD_MC && method != DTFE_METHOD_EQUI)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "method must be 0 (centre), 2 (Monte Carlo) or 3 (equidistant)");
+    if (method != DTFE_METHOD_CENTRE && mc->samples < 1)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "samples per cell must be >= 1");
+    if (method == DTFE_METHOD_EQUI && smp->kind != DTFE_SAMPLE_GRID)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "equidistant averaging is only available on a regular grid");
+    if ((fields & DTFE_FIELD_VELOCITY_STD) && !(smp->kind == DTFE_SAMPLE_GRID && method == DTFE_METHOD_MC))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "velocity_std needs Monte-Carlo averaging on a regular grid");
+    HIP_TRY(hipSetDevice(ctx->device));
+
+    SampleArgs a{};
+    a.pos = m->pos;
+    a.cells = (const int4 *)m->cells;
+    a.nbrs = (const int4 *)m->neighbors;
+    a.nc = m->n_cells;
+    a.vf = VertexFieldsDev{vf->density, vf->velocity, vf->scalar, vf->n_scalar};
+    a.fields = fields;
+    a.flags = flags;
+    a.kind = smp->kind;
+    a.method = method;
+    a.M = method == DTFE_METHOD_CENTRE ? 1 : mc->samples;
+    a.arg_order = mc->arg_order;
+    a.out = OutPtrs{out->density, out->velocity, out->velocity_gradient, out->velocity_divergence,
+                    out->velocity_shear, out->velocity_vorticity, out->velocity_std, out->scalar,
+                    out->scalar_gradient, out->located_cell};
+    const size_t esz = vf->dtype == DTFE_F64 ? 8 : 4;
+
+    // ---- rows and their coordinates ------------------------------------------------
+    int64_t n_rows = 0;
+    std::vector<double> axes;
+    std::vector<double> trig;
+    if (smp->kind == DTFE_SAMPLE_GRID || smp->kind == DTFE_SAMPLE_CONE) {
+        for (int i = 0; i < 3; ++i)
+            if (smp->n[i] < 1) return fail(ctx, DTFE_ERR_ARGUMENT, "grid sizes must be >= 1");
+        n_rows = smp->n[0] * smp->n[1] * smp->n[2];
+        double bx[6];
+        for (int i = 0; i < 6; ++i) bx[i] = smp->kind == DTFE_SAMPLE_GRID ? smp->box[i] : smp->cone[i];
+        if (smp->kind == DTFE_SAMPLE_CONE) {
+            // checkAngles (triangulation_miscellaneous.cc:162-199), degrees -> radians with PI = 3.14159265
+            if (bx[2] < 0 || bx[2] > 180 || bx[3] < 0 || bx[3] > 180)
+                return fail(ctx, DTFE_ERR_ARGUMENT, "cone theta limits must be in [0, 180] degrees");
+            if (bx[5] - bx[4] >= 360.0) return fail(ctx, DTFE_ERR_ARGUMENT, "cone psi range must be < 360 degrees");
+            double *ang = bx + 2;
+            ang[2] -= std::floor(ang[2] / 360.) * 360.;
+            ang[3] -= std::floor(ang[3] / 360.) * 360.;
+            if (ang[2] > ang[3]) ang[3] += 360.;
+            const double deg = 3.14159265 / 180.;
+            for (int i = 0; i < 4; ++i) ang[i] *= deg;
+        }
+        for (int i = 0; i < 3; ++i) a.dx[i] = (bx[2 * i + 1] - bx[2 * i]) / (double)smp->n[i];
+        // centres by the reference's running sums (x = x0 + dx/2; x += dx)
+        for (int ax = 0; ax < 3; ++ax) {
+            double x = bx[2 * ax] + a.dx[ax] / 2.;
+            for (int64_t i = 0; i < smp->n[ax]; ++i) {
+                axes.push_back(x);
+                x += a.dx[ax];
+            }
+        }
+        if (smp->kind == DTFE_SAMPLE_CONE) {
+            const double *th = axes.data() + smp->n[0];
+            const double *ps = th + smp->n[1];
+            for (int64_t j = 0; j < smp->n[1]; ++j) trig.push_back(std::sin(th[j]));
+            for (int64_t j = 0; j < smp->n[1]; ++j) trig.push_back(std::cos(th[j]));
+            for (int64_t k = 0; k < smp->n[2]; ++k) trig.push_back(std::sin(ps[k]));
+            for (int64_t k = 0; k < smp->n[2]; ++k) trig.push_back(std::cos(ps[k]));
+            for (int i = 0; i < 3; ++i) a.origin[i] = smp->origin[i];
+        }
+        for (int i = 0; i < 3; ++i) a.n[i] = smp->n[i];
+    } else if (smp->kind == DTFE_SAMPLE_POINTS) {
+        if (smp->n_points < 0) return fail(ctx, DTFE_ERR_ARGUMENT, "negative point count");
+        if (smp->n_points > 0 && !smp->points) return fail(ctx, DTFE_ERR_ARGUMENT, "points is NULL");
+        if (method == DTFE_METHOD_MC && smp->n_points > 0 && !smp->deltas)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "Monte-Carlo user sampling needs deltas");
+        a.n[0] = a.n[1] = a.n[2] = 1;
+        a.pts = smp->points;
+        a.deltas = smp->deltas;
+        // region filter + compaction (host round trip of one flag per point)
+        int64_t np = smp->n_points;
+        const size_t box_off = ((size_t)std::max<int64_t>(np, 1) * sizeof(int) + 255) & ~(size_t)255;
+        if ((rc = ensure(ctx, ctx->flags, box_off + 6 * sizeof(double)))) return rc;
+        double *dbox = (double *)((char *)ctx->flags.p + box_off);
+        HIP_TRY(hipMemcpyAsync(dbox, smp->box, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        std::vector<int> fl(np);
+        if (np > 0) {
+            hipLaunchKernelGGL(k_points_in_box, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, smp->points, np,
+                               dbox, (int *)ctx->flags.p);
+            HIP_TRY(hipMemcpyAsync(fl.data(), ctx->flags.p, np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+        std::vector<int64_t> rows;
+        rows.reserve(np);
+        for (int64_t i = 0; i < np; ++i)
+            if (fl[i]) rows.push_back(i);
+        n_rows = (int64_t)rows.size();
+        if ((rc = ensure(ctx, ctx->rows, std::max<int64_t>(n_rows, 1) * sizeof(int64_t)))) return rc;
+        if (n_rows)
+            HIP_TRY(hipMemcpyAsync(ctx->rows.p, rows.data(), n_rows * sizeof(int64_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        a.row_point = (const int64_t *)ctx->rows.p;
+        a.n[0] = n_rows > 0 ? n_rows : 1;
+    } else {
+        return fail(ctx, DTFE_ERR_ARGUMENT, "unknown sampling kind");
+    }
+    a.n_rows = n_rows;
+    out->n_written = n_rows;
+    if (n_rows == 0) return DTFE_OK;
+
+    // fewer than four vertices: zero-filled fields (isTriangulationIncomplete)
+    if (m->n_vertices < 4 || m->n_cells == 0) {
+        void *ptrs[] = {out->density, out->velocity, out->velocity_gradient, out->velocity_divergence,
+                        out->velocity_shear, out->velocity_vorticity, out->velocity_std, out->scalar,
+                        out->scalar_gradient};
+        unsigned bits[] = {DTFE_FIELD_DENSITY, DTFE_FIELD_VELOCITY, DTFE_FIELD_VELOCITY_GRADIENT,
+                           DTFE_FIELD_VELOCITY_DIVERGENCE, DTFE_FIELD_VELOCITY_SHEAR, DTFE_FIELD_VELOCITY_VORTICITY,
+                           DTFE_FIELD_VELOCITY_STD, DTFE_FIELD_SCALAR, DTFE_FIELD_SCALAR_GRADIENT};
+        size_t comps[] = {1, 3, 9, 1, 5, 3, 1, (size_t)std::max(1, vf->n_scalar), (size_t)3 * std::max(1, vf->n_scalar)};
+        for (int i = 0; i < 9; ++i)
+            if ((fields & bits[i]) && ptrs[i]) {
+                int64_t cnt = n_rows * (int64_t)comps[i];
+                if (esz == 8)
+                    hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(cnt)), dim3(kBlock), 0, ctx->stream,
+                                       (double *)ptrs[i], cnt, 0.0);
+                else
+                    hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(cnt)), dim3(kBlock), 0, ctx->stream,
+                                       (float *)ptrs[i], cnt, 0.0f);
+            }
+        if (out->located_cell)
+            hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(n_rows)), dim3(kBlock), 0, ctx->stream,
+                               out->located_cell, n_rows, -1);
+        return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: fields are zero");
+    }
+
+    // ---- tables ------------------------------------------------------------------------
+    if (!axes.empty()) {
+        if ((rc = ensure(ctx, ctx->axes, axes.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->axes.p, axes.data(), axes.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.ax0 = (const double *)ctx->axes.p;
+        a.ax1 = a.ax0 + a.n[0];
+        a.ax2 = a.ax1 + a.n[1];
+    }
+    if (!trig.empty()) {
+        if ((rc = ensure(ctx, ctx->trig, trig.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->trig.p, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.trig = (const double *)ctx->trig.p;
+    }
+    if (method == DTFE_METHOD_EQUI) {
+        int NN = mc->samples;
+        int n3 = (int)std::lround(std::cbrt((double)NN));
+        if (n3 * n3 * n3 != NN) return fail(ctx, DTFE_ERR_ARGUMENT, "equidistant averaging needs a perfect cube of samples");
+        std::vector<double> sp(3 * (size_t)NN);
+        for (int i1 = 0; i1 < n3; ++i1)
+            for (int i2 = 0; i2 < n3; ++i2)
+                for (int i3 = 0; i3 < n3; ++i3) {
+                    int idx = i1 * n3 * n3 + i2 * n3 + i3;
+                    // averaged_interpolation_2.cc:725-740
+                    sp[3 * idx + 2] = (i3 + 0.5) * a.dx[2] / n3 - a.dx[2] / 2.;
+                    sp[3 * idx + 0] = (i1 + 0.5) * a.dx[0] / n3 - a.dx[0] / 2.;
+                    sp[3 * idx + 1] = (i2 + 0.5) * a.dx[1] / n3 - a.dx[1] / 2.;
+                }
+        if ((rc = ensure(ctx, ctx->equi, sp.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->equi.p, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        a.equi = (const double *)ctx->equi.p;
+    }
+
+    // ---- seed grid + status --------------------------------------------------------------
+    if ((rc = build_seed_grid(ctx, m, a.sg))) return rc;
+    if ((rc = ensure(ctx, ctx->status, 16))) return rc;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->status.p, (int64_t)4, 0);
+    a.status = (int *)ctx->status.p;
+
+    // ---- launch -------------------------------------------------------------------------------
+    const int ns = want_scal ? vf->n_scalar : 0;
+    int grid;
+    size_t lds = 0;
+    int is_mc = method == DTFE_METHOD_MC;
+    if (is_mc) {
+        const int64_t wpr = 3 * (int64_t)a.M;
+        int rpr = kBlock;
+        // ring must hold every MT block one round touches plus the previous one
+        while (rpr > 1 && ((wpr * rpr + MT_N - 1) / MT_N + 2) * MT_N * 4 > 48 * 1024) rpr /= 2;
+        a.rows_per_round = rpr;
+        a.ring = (int)((wpr * rpr + MT_N - 1) / MT_N + 2);
+        lds = (size_t)a.ring * MT_N * sizeof(uint32_t);
+        int64_t rounds = std::max<int64_t>(1, std::min<int64_t>(16, n_rows / ((int64_t)rpr * 2048)));
+        a.rows_per_block = rpr * rounds;
+        int64_t nblk = (n_rows + a.rows_per_block - 1) / a.rows_per_block;
+        grid = (int)nblk;
+        uint64_t s = mc->seed;
+        if (mc->part_no >= 0) s += 1000ull * (uint64_t)mc->part_no;
+        uint32_t seed32 = (uint32_t)s;
+        if (!(ctx->plan_valid && ctx->plan_seed == seed32 && ctx->plan_wpr == wpr && ctx->plan_rows == n_rows &&
+              ctx->plan_rpb == a.rows_per_block)) {
+            std::vector<int64_t> first(nblk);
+            for (int64_t b = 0; b < nblk; ++b) first[b] = (wpr * (b * a.rows_per_block)) / MT_N;
+            mt_build_plan(seed32, first, ctx->plan_host);
+            if ((rc = ensure(ctx, ctx->plan, ctx->plan_host.size() * sizeof(uint32_t)))) return rc;
+            HIP_TRY(hipMemcpyAsync(ctx->plan.p, ctx->plan_host.data(), ctx->plan_host.size() * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            ctx->plan_valid = true;
+            ctx->plan_seed = seed32;
+            ctx->plan_wpr = wpr;
+            ctx->plan_rows = n_rows;
+            ctx->plan_rpb = a.rows_per_block;
+        }
+        a.plan = (const uint32_t *)ctx->plan.p;
+    } else {
+        grid = grid_for(n_rows);
+    }
+    // rows whose walk meets a sign the FP64 filter cannot certify are queued
+    // for the exact kernel, with their MC words
+    const int64_t wpr = is_mc ? 3 * (int64_t)a.M : 0;
+    int64_t cap = std::min<int64_t>(n_rows, std::max<int64_t>(1 << 16, n_rows / 16));
+    if ((rc = ensure(ctx, ctx->defer_rows, cap * sizeof(int64_t)))) return rc;
+    if (wpr && (rc = ensure(ctx, ctx->defer_words, cap * wpr * sizeof(uint32_t)))) return rc;
+    a.defer_rows = (int64_t *)ctx->defer_rows.p;
+    a.defer_words = (uint32_t *)ctx->defer_words.p;
+    a.defer_cap = (int)cap;
+    const bool f64 = vf->dtype == DTFE_F64;
+    launch_any(a, f64, want_vel, ns, 0, is_mc, grid, lds, 0, ctx->stream);
+    HIP_TRY(hipGetLastError());
+    int st[4];
+    HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->last_deferred = st[1];
+    if (st[2]) {
+        // queue overflow: redo everything with exact signs
+        launch_any(a, f64, want_vel, ns, 2, is_mc, grid, lds, 0, ctx->stream);
+    } else if (st[1] > 0) {
+        launch_any(a, f64, want_vel, ns, 1, is_mc, grid_for(st[1]), 0, st[1], ctx->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    if (st[1] > 0 || st[2]) {
+        HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    if (st[0] > 0)
+        return fail(ctx, DTFE_ERR_WALK, std::to_string(st[0]) + " point locations did not terminate: the mesh is not a "
+                                                              "valid triangulation");
+    return DTFE_OK;
+}
+
+int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, int dtype, const void *weight,
+                     const void *velocity, const double box[6], const int64_t n[3], double rho_bar, unsigned fields,
+                     dtfe_outputs *out) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    if (scheme != 0 && scheme != 1) return fail(ctx, DTFE_ERR_ARGUMENT, "scheme must be 0 (CIC) or 1 (TSC)");
+    if (dtype != DTFE_F32 && dtype != DTFE_F64) return fail(ctx, DTFE_ERR_ARGUMENT, "bad dtype");
+    if (!box || !n || !out) return fail(ctx, DTFE_ERR_ARGUMENT, "NULL argument");
+    for (int i = 0; i < 3; ++i)
+        if (n[i] < 2) return fail(ctx, DTFE_ERR_ARGUMENT, "deposit grids need at least 2 cells per axis");
+    if (np < 0 || (np > 0 && (!pos || !weight))) return fail(ctx, DTFE_ERR_ARGUMENT, "bad particle arrays");
+    const bool want_v = (fields & DTFE_FIELD_VELOCITY) != 0;
+    const bool want_d = (fields & DTFE_FIELD_DENSITY) != 0;
+    if (want_v && (np > 0 && !velocity)) return fail(ctx, DTFE_ERR_ARGUMENT, "velocity requested without velocities");
+    if ((want_v && !out->velocity) || (want_d && !out->density))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "output array is NULL");
+    if (!(rho_bar > 0.0)) return fail(ctx, DTFE_ERR_ARGUMENT, "average density must be positive");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t ncell = n[0] * n[1] * n[2];
+    const size_t esz = dtype == DTFE_F64 ? 8 : 4;
+    int rc;
+    if ((rc = ensure(ctx, ctx->mass, ncell * esz))) return rc;
+    if ((rc = ensure(ctx, ctx->mom, 3 * ncell * esz))) return rc;
+    // cleared with a kernel rather than a memset node, so the call can be graph-captured
+    if (dtype == DTFE_F64) {
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(ncell)), dim3(kBlock), 0, ctx->stream, (double *)ctx->mass.p,
+                           ncell, 0.0);
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(3 * ncell)), dim3(kBlock), 0, ctx->stream,
+                           (double *)ctx->mom.p, 3 * ncell, 0.0);
+    } else {
+        hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(ncell)), dim3(kBlock), 0, ctx->stream, (float *)ctx->mass.p,
+                           ncell, 0.0f);
+        hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(3 * ncell)), dim3(kBlock), 0, ctx->stream,
+                           (float *)ctx->mom.p, 3 * ncell, 0.0f);
+    }
+    DepositArgs a{};
+    a.pos = pos;
+    a.w = weight;
+    a.vel = velocity;
+    a.np = np;
+    a.scheme = scheme;
+    a.want_vel = want_v ? 1 : 0;
+    for (int i = 0; i < 3; ++i) {
+        a.n[i] = (int)n[i];
+        a.dx[i] = (box[2 * i + 1] - box[2 * i]) / (double)n[i];
+    }
+    for (int i = 0; i < 6; ++i) a.box[i] = box[i];
+    for (int i = 0; i < 3; ++i) {
+        a.outer[2 * i] = box[2 * i] - a.dx[i];
+        a.outer[2 * i + 1] = box[2 * i + 1] + a.dx[i];
+        a.inner[2 * i] = box[2 * i] - (-1.1 * a.dx[i]);
+        a.inner[2 * i + 1] = box[2 * i + 1] + (-1.1 * a.dx[i]);
+    }
+    a.mass = ctx->mass.p;
+    a.mom = ctx->mom.p;
+    if (np > 0) {
+        if (dtype == DTFE_F64)
+            hipLaunchKernelGGL(k_deposit<double>, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL(k_deposit<float>, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, a);
+    }
+    double factor = (double)ncell / (35000. * 35000. * 35000.) / rho_bar;
+    if (dtype == DTFE_F64)
+        hipLaunchKernelGGL(k_deposit_finish<double>, dim3(grid_for(ncell)), dim3(kBlock), 0, ctx->stream,
+                           (const double *)ctx->mass.p, (const double *)ctx->mom.p, ncell, factor,
+                           want_d ? (double *)out->density : nullptr, want_v ? (double *)out->velocity : nullptr);
+    else
+        hipLaunchKernelGGL(k_deposit_finish<float>, dim3(grid_for(ncell)), dim3(kBlock), 0, ctx->stream,
+                           (const float *)ctx->mass.p, (const float *)ctx->mom.p, ncell, (float)factor,
+                           want_d ? (float *)out->density : nullptr, want_v ? (float *)out->velocity : nullptr);
+    HIP_TRY(hipGetLastError());
+    out->n_written = ncell;
+    return DTFE_OK;
+}
+
+// Rows of the last sampling call that needed exact signs.
+int dtfe_hip_last_deferred(dtfe_ctx *ctx) { return ctx ? ctx->last_deferred : -1; }
+
+// Number of walks that did not terminate in the last sampling call
+// (synchronises the stream). Not part of the drop-in surface.
+int dtfe_hip_walk_failures(dtfe_ctx *ctx, int *failures) {
+    if (!ctx || !failures) return DTFE_ERR_ARGUMENT;
+    if (!ctx->status.p) {
+        *failures = 0;
+        return DTFE_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(failures, ctx->status.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return DTFE_OK;
+}
+
+}  // extern "C"

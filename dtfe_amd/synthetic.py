@@ -1,0 +1,117 @@
+"""Seeded synthetic particle sets for the DTFE workloads.
+
+The reference ships no data (its demo snapshot ``demo/z0_64.gadget`` is
+missing), so every workload is generated here from numpy's PCG64 with a fixed
+seed.  The recipe: a periodic cube [0, L)^3; a fraction of the particles is
+uniform in the box, the rest is split with equal probability among Gaussian
+blobs whose centres are uniform in the box and whose widths are log-uniform in
+[sigma_lo, sigma_hi]; blob positions wrap periodically.
+
+Workloads (sizes as named by the benchmark configurations):
+
+* ``clustered_density``   N = 64^3,  L = 100, 200 blobs, sigma in [0.5, 3],
+  unit mass (density on a 64^3 cell-centre grid).
+* ``clustered_velocity``  N = 128^3, L = 200, 2000 blobs, sigma in [0.5, 3];
+  velocity = 100 (x_unwrapped - c_blob) + N(0, 300^2) per component (uniform
+  particles: dispersion only); weights lognormal(0, 0.3)
+  (density, velocity and velocity gradient on a 256^3 grid, 8 MC samples).
+* ``clustered_scalars``   N = 256^3, L = 400, 20000 blobs, sigma in [0.2, 2],
+  unit mass, 3 scalars: z ~ N(0, 1) per particle, replaced by the blob mean
+  of z for blob particles.
+* ``cone_survey``         N = 4e6, L = 800, non-periodic, weights
+  proportional to exp(-r / 300) with r measured from the corner (0, 0, 0).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class ParticleSet:
+    pos: np.ndarray                 # [N, 3] float64
+    weight: np.ndarray              # [N] float64
+    vel: Optional[np.ndarray] = None     # [N, 3] float64
+    scalar: Optional[np.ndarray] = None  # [N, S] float64
+    box: tuple = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    periodic: bool = True
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+
+def clustered(n: int, box_len: float, n_blobs: int, sigma_lo: float, sigma_hi: float,
+              seed: int, with_velocity: bool = False, lognormal_weights: bool = False,
+              n_scalars: int = 0, uniform_fraction: float = 0.5) -> ParticleSet:
+    rng = np.random.default_rng(seed)
+    n_uniform = int(round(n * uniform_fraction))
+    n_blob = n - n_uniform
+    centres = rng.random((n_blobs, 3)) * box_len
+    sigmas = np.exp(rng.uniform(np.log(sigma_lo), np.log(sigma_hi), n_blobs))
+    member = rng.integers(0, n_blobs, n_blob)
+    offsets = rng.standard_normal((n_blob, 3)) * sigmas[member, None]
+    blob_pos = centres[member] + offsets
+    uni_pos = rng.random((n_uniform, 3)) * box_len
+    pos = np.concatenate([uni_pos, np.mod(blob_pos, box_len)], axis=0)
+    # np.mod can return box_len for tiny negative inputs; keep [0, L)
+    pos[pos >= box_len] -= box_len
+    ps = ParticleSet(pos=pos.astype(np.float64), weight=np.ones(n, dtype=np.float64),
+                     box=(0.0, box_len, 0.0, box_len, 0.0, box_len), periodic=True,
+                     meta=dict(n_blobs=n_blobs, seed=seed, sigma=(sigma_lo, sigma_hi)))
+    if with_velocity:
+        disp = rng.standard_normal((n, 3)) * 300.0
+        vel = disp.copy()
+        vel[n_uniform:] += 100.0 * offsets
+        ps.vel = vel
+    if lognormal_weights:
+        ps.weight = rng.lognormal(0.0, 0.3, n)
+    if n_scalars > 0:
+        z = rng.standard_normal((n, n_scalars))
+        sums = np.zeros((n_blobs, n_scalars))
+        counts = np.bincount(member, minlength=n_blobs).astype(np.float64)
+        for k in range(n_scalars):
+            sums[:, k] = np.bincount(member, weights=z[n_uniform:, k], minlength=n_blobs)
+        means = sums / np.maximum(counts, 1.0)[:, None]
+        z[n_uniform:] = means[member]
+        ps.scalar = z
+    return ps
+
+
+def clustered_density(seed: int = 1, n_side: int = 64) -> ParticleSet:
+    return clustered(n_side ** 3, 100.0, 200, 0.5, 3.0, seed)
+
+
+def clustered_velocity(seed: int = 2, n_side: int = 128, box_len: float = 200.0,
+                       n_blobs: int = 2000) -> ParticleSet:
+    return clustered(n_side ** 3, box_len, n_blobs, 0.5, 3.0, seed, with_velocity=True,
+                     lognormal_weights=True)
+
+
+def clustered_scalars(seed: int = 3, n_side: int = 256) -> ParticleSet:
+    return clustered(n_side ** 3, 400.0, 20000, 0.2, 2.0, seed, n_scalars=3)
+
+
+def cone_survey(seed: int = 5, n: int = 4_000_000, box_len: float = 800.0) -> ParticleSet:
+    ps = clustered(n, box_len, max(1, n // 1000), 0.5, 3.0, seed, with_velocity=True)
+    r = np.linalg.norm(ps.pos, axis=1)
+    ps.weight = np.exp(-r / 300.0)
+    ps.periodic = False
+    return ps
+
+
+def small_random(n: int, seed: int, box_len: float = 10.0, with_velocity: bool = True,
+                 n_scalars: int = 0) -> ParticleSet:
+    """Uniform particles for fast parity tests."""
+    rng = np.random.default_rng(seed)
+    ps = ParticleSet(pos=rng.random((n, 3)) * box_len, weight=rng.lognormal(0.0, 0.3, n),
+                     box=(0.0, box_len) * 3, periodic=True)
+    ps.box = (0.0, box_len, 0.0, box_len, 0.0, box_len)
+    if with_velocity:
+        ps.vel = rng.standard_normal((n, 3)) * 10.0
+    if n_scalars:
+        ps.scalar = rng.standard_normal((n, n_scalars))
+    return ps

@@ -1,0 +1,166 @@
+/*
+ * dtfe_hip.h — C ABI of the MI355X (gfx950) DTFE interpolation path.
+ *
+ * Everything the reference does after building its Delaunay tessellation
+ * runs behind these entry points: vertex densities, per-tetrahedron linear
+ * fields and their gradients, point location and sampling (cell centres,
+ * Monte-Carlo and equidistant cell averages, redshift cones, user points),
+ * derived velocity fields, CIC/TSC deposits and the adjoint.
+ *
+ * Conventions
+ *  - Plain C types only. Array pointers inside the argument structs are
+ *    DEVICE pointers (memory the caller allocated on the context's device);
+ *    the structs themselves live in host memory.
+ *  - Every call returns a status (DTFE_OK = 0) and never exits the process;
+ *    dtfe_hip_last_error() gives the message of the last failure.
+ *  - Calls are asynchronous on the context's stream unless stated otherwise;
+ *    calls on one context are serialised, contexts are independent.
+ *  - Meshes are "flattened Delaunay tessellations": positions [nv][3] in
+ *    FP64, finite tetrahedra [nc][4] (vertex ids, positively oriented), and
+ *    for each tetrahedron the neighbour across the face opposite vertex i,
+ *    or -1 when that face lies on the convex hull (the reference's infinite
+ *    cells). Vertex 0 of a tetrahedron is the reference's "base" vertex.
+ */
+#ifndef DTFE_HIP_H
+#define DTFE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DTFE_OK 0
+#define DTFE_ERR_ARGUMENT 1        /* invalid argument (bad size, method, null pointer) */
+#define DTFE_ERR_HIP 2             /* a HIP runtime call failed */
+#define DTFE_ERR_NO_MEMORY 3       /* device allocation failed */
+#define DTFE_ERR_TOO_FEW_VERTICES 4/* fewer than 4 vertices: fields are zero-filled */
+#define DTFE_ERR_WALK 5            /* point location did not terminate (mesh is not a triangulation) */
+
+/* ---- precision of field data and outputs --------------------------------- */
+#define DTFE_F32 0   /* mixed precision: FP64 geometry, FP32 field arithmetic */
+#define DTFE_F64 1   /* all FP64 */
+
+/* ---- field selection (the reference's Field, src/user_options.h:48-97) ---- */
+#define DTFE_FIELD_DENSITY             (1u << 0)
+#define DTFE_FIELD_VELOCITY            (1u << 1)
+#define DTFE_FIELD_VELOCITY_GRADIENT   (1u << 2)
+#define DTFE_FIELD_VELOCITY_DIVERGENCE (1u << 3)
+#define DTFE_FIELD_VELOCITY_SHEAR      (1u << 4)
+#define DTFE_FIELD_VELOCITY_VORTICITY  (1u << 5)
+#define DTFE_FIELD_VELOCITY_STD        (1u << 6)
+#define DTFE_FIELD_SCALAR              (1u << 7)
+#define DTFE_FIELD_SCALAR_GRADIENT     (1u << 8)
+
+/* ---- behaviour switches --------------------------------------------------- */
+/* Keep the reference's velocity/scalar value formula, which keeps only the
+ * z term (src/CGAL_triangulation/unaveraged_interpolation.cc:116, :159).
+ * Off: the linear interpolant sum_i lambda_i f_i. */
+#define DTFE_COMPAT_VALUE_BUG          (1u << 0)
+/* Divide Monte-Carlo sums by the number of samples inside the hull instead
+ * of by M (the reference divides by M, averaged_interpolation_2.cc:195). */
+#define DTFE_AVERAGE_FINITE_ONLY       (1u << 1)
+
+/* ---- sampling kinds and methods ------------------------------------------------ */
+#define DTFE_SAMPLE_GRID   0   /* regular grid over box[6], n[3] cells, row-major, z fastest */
+#define DTFE_SAMPLE_CONE   1   /* redshift cone: cone[6] = r, theta, psi ranges (degrees) */
+#define DTFE_SAMPLE_POINTS 2   /* user points with per-point cell sizes (delta) */
+
+#define DTFE_METHOD_CENTRE 0   /* value at the sample point (the reference's uField) */
+#define DTFE_METHOD_MC     2   /* Monte-Carlo average over the cell (averaging method 2) */
+#define DTFE_METHOD_EQUI   3   /* equidistant n^3 average (averaging method 3, grid only) */
+
+typedef struct dtfe_ctx dtfe_ctx;
+
+typedef struct {
+    int64_t n_vertices;
+    int64_t n_cells;
+    const double  *pos;        /* [n_vertices][3] */
+    const int32_t *cells;      /* [n_cells][4] */
+    const int32_t *neighbors;  /* [n_cells][4] */
+} dtfe_mesh;
+
+typedef struct {
+    int dtype;               /* DTFE_F32 or DTFE_F64: type of every array below */
+    const void *density;     /* [nv] vertex density (from dtfe_hip_vertex_density) */
+    const void *velocity;    /* [nv][3] or NULL */
+    const void *scalar;      /* [nv][n_scalar] or NULL */
+    int n_scalar;
+} dtfe_vertex_fields;
+
+typedef struct {
+    int kind;                /* DTFE_SAMPLE_* */
+    double box[6];           /* grid region / region filter for user points: xmin,xmax,ymin,ymax,zmin,zmax */
+    int64_t n[3];            /* grid or cone bins */
+    double cone[6];          /* r_min, r_max, theta_min, theta_max, psi_min, psi_max (degrees) */
+    double origin[3];        /* cone origin */
+    int64_t n_points;        /* user points */
+    const double *points;    /* [n_points][3] device */
+    const double *deltas;    /* [n_points][3] device (Monte-Carlo cell size per point) */
+} dtfe_sampling;
+
+typedef struct {
+    int method;              /* DTFE_METHOD_* */
+    int samples;             /* samples per cell (MC) or n^3 (equidistant) */
+    uint64_t seed;           /* randomSeed; the stream is mt19937 seeded with seed + 1000*part_no */
+    int part_no;             /* -1 for a single partition */
+    int arg_order;           /* grid MC draw order: 0 = x,y,z (clang), 1 = z,y,x */
+} dtfe_mc;
+
+typedef struct {
+    void *density;             /* [n]       */
+    void *velocity;            /* [n][3]    */
+    void *velocity_gradient;   /* [n][9]    grad[i*3+j] = d v_i / d x_j */
+    void *velocity_divergence; /* [n]       */
+    void *velocity_shear;      /* [n][5]    */
+    void *velocity_vorticity;  /* [n][3]    */
+    void *velocity_std;        /* [n]       */
+    void *scalar;              /* [n][S]    */
+    void *scalar_gradient;     /* [n][3S]   */
+    int32_t *located_cell;     /* optional [n]: cell holding the sample point (centre), -1 outside the hull */
+    int64_t n_written;         /* set on return: rows written (user points outside box[] give no row) */
+} dtfe_outputs;
+
+/* ---- context ---------------------------------------------------------------- */
+int dtfe_hip_create(int device, dtfe_ctx **out);
+int dtfe_hip_destroy(dtfe_ctx *ctx);
+/* Run subsequent work on this HIP stream; NULL selects the device's null
+ * stream. A new context runs on a stream of its own. */
+int dtfe_hip_set_stream(dtfe_ctx *ctx, void *hip_stream);
+int dtfe_hip_synchronize(dtfe_ctx *ctx);
+const char *dtfe_hip_last_error(dtfe_ctx *ctx);
+
+/* Vertex density, replacing vertexDensity() (src/CGAL_triangulation/triangulation.cpp:200-279):
+ * W_v = sum of the volumes of the finite tetrahedra around v,
+ * rho_v = 4 w_v / (rho_bar W_v), and 0 on hull vertices.
+ * weight and rho_out are [nv] of `dtype`. */
+int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *mesh, int dtype, const void *weight,
+                            double rho_bar, void *rho_out);
+
+/* Sampling, replacing interpolateGrid / interpolateRedshiftCone /
+ * interpolateUserSampling (src/CGAL_triangulation/unaveraged_interpolation.cc:209-691)
+ * and their averaged forms (src/CGAL_triangulation/averaged_interpolation_2.cc:35-866),
+ * with the derived fields of computeDivergenceShearVorticity (src/DTFE.cpp:436-475)
+ * fused. `fields` is a mask of DTFE_FIELD_*; `flags` of DTFE_COMPAT_* / DTFE_AVERAGE_*. */
+int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *mesh, const dtfe_vertex_fields *vf,
+                    unsigned fields, const dtfe_sampling *sampling, const dtfe_mc *mc,
+                    unsigned flags, dtfe_outputs *out);
+
+/* CIC (scheme 0) and TSC (scheme 1) deposits, replacing CIC_interpolation_regular_grid
+ * (src/CIC_interpolation.cc:67-273) and TSC_interpolation_regular_grid
+ * (src/TSC_interpolation.cc:67-254). pos [np][3] FP64; weight [np], velocity [np][3]
+ * of `dtype`; outputs density [n] and velocity [n][3] of `dtype`. `fields` selects
+ * DTFE_FIELD_DENSITY / DTFE_FIELD_VELOCITY. */
+int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, int dtype,
+                     const void *weight, const void *velocity, const double box[6],
+                     const int64_t n[3], double rho_bar, unsigned fields, dtfe_outputs *out);
+
+/* Library version string. */
+const char *dtfe_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTFE_HIP_H */

@@ -1,0 +1,698 @@
+// TEST INFRASTRUCTURE — the CPU FP64 oracle of the DTFE interpolation path.
+//
+// Only tests/, the smoke check and the CPU-baseline leg of the benchmark may
+// load this library; the product path never does.
+//
+// It restates, in plain serial C++ on a flattened Delaunay mesh, what the
+// reference computes after its CGAL tessellation, with Real = double
+// (the reference's -DDOUBLE build, src/define.h:23-27):
+//   * vertex density ........ src/CGAL_triangulation/triangulation.cpp:200-279
+//   * tetrahedron inverse ... src/CGAL_triangulation/triangulation_miscellaneous.cc:42-96
+//   * per-sample fields ..... src/CGAL_triangulation/unaveraged_interpolation.cc:34-173
+//   * cell-centre sampling .. src/CGAL_triangulation/unaveraged_interpolation.cc:209-691
+//   * Monte-Carlo / equidistant averaging
+//                            src/CGAL_triangulation/averaged_interpolation_2.cc:35-866
+//   * standard deviation .... src/CGAL_triangulation/triangulation_miscellaneous.cc:242-271
+//   * angle conversion ...... src/CGAL_triangulation/triangulation_miscellaneous.cc:162-199
+//   * div / shear / vort .... src/DTFE.cpp:394-475
+//   * CIC / TSC deposits .... src/CIC_interpolation.cc:67-273, src/TSC_interpolation.cc:67-254
+//   * periodic padding ...... src/subpartition.h:79-154
+//   * average density ....... src/subpartition.h:437-449, src/box.h:141-148
+// Point location is a visibility walk with exact signs (exact.h); like
+// CGAL's locate it starts from the previously located cell.
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include <array>
+
+#include "exact.h"
+
+using oracle_exact::orient_sos;
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// Mersenne twister MT19937 (the engine of boost::mt19937, which the reference
+// seeds at averaged_interpolation_2.cc:96-102) and boost's uniform_real
+// mapping for a 32-bit integer engine: u = x / 2^32 * (b - a) + a.
+// ----------------------------------------------------------------------------
+struct MT19937 {
+    uint32_t mt[624];
+    int idx;
+    explicit MT19937(uint32_t seed) {
+        mt[0] = seed;
+        for (int i = 1; i < 624; ++i)
+            mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+        idx = 624;
+    }
+    void twist() {
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = (mt[i] & 0x80000000u) | (mt[(i + 1) % 624] & 0x7fffffffu);
+            uint32_t v = mt[(i + 397) % 624] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            mt[i] = v;
+        }
+        idx = 0;
+    }
+    uint32_t next() {
+        if (idx >= 624) twist();
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+};
+
+struct Uniform {
+    MT19937 gen;
+    explicit Uniform(uint64_t seed) : gen((uint32_t)seed) {}
+    double operator()() {
+        double numerator = (double)gen.next();
+        double divisor = 4294967295.0 + 1.0;
+        return numerator / divisor * (0.5 - (-0.5)) + (-0.5);
+    }
+};
+
+struct Mesh {
+    int64_t nv;
+    const double *pos;      // [nv][3]
+    int64_t nc;
+    const int32_t *cells;   // [nc][4] vertex ids, positively oriented
+    const int32_t *nbrs;    // [nc][4] neighbour across the face opposite vertex i, -1 = hull
+};
+
+// Faces written so that orient(face, p) > 0 when p is on the inner side:
+// O_i(p) = det of the tetrahedron with vertex i replaced by p.
+static const int FACE[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
+
+static inline const double *P(const Mesh &m, int v) { return m.pos + 3 * (int64_t)v; }
+
+// Visibility walk from `start`; returns the containing cell or -1 when the
+// point is outside the convex hull (the reference's "infinite cell").
+int64_t locate(const Mesh &m, const double p[3], int64_t start, int64_t *steps_out) {
+    int64_t t = start < 0 ? 0 : start;
+    int64_t prev = -1;
+    int64_t steps = 0;
+    const int64_t max_steps = 4 * m.nc + 16;
+    while (steps++ < max_steps) {
+        const int32_t *c = m.cells + 4 * t;
+        const int32_t *nb = m.nbrs + 4 * t;
+        int moved = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (nb[i] >= 0 && nb[i] == prev) continue;
+            double val;
+            int s = orient_sos(P(m, c[FACE[i][0]]), P(m, c[FACE[i][1]]), P(m, c[FACE[i][2]]), p, &val);
+            if (s < 0) {
+                if (nb[i] < 0) {
+                    if (steps_out) *steps_out = steps;
+                    return -1;
+                }
+                prev = t;
+                t = nb[i];
+                moved = 1;
+                break;
+            }
+        }
+        if (!moved) {
+            if (steps_out) *steps_out = steps;
+            return t;
+        }
+    }
+    if (steps_out) *steps_out = -1;
+    return -2;  // walk did not terminate: mesh is not a valid triangulation
+}
+
+// ---- reference matrix inverse with its degeneracy rule ----------------------
+static inline double minorDet(const double A[3][3], int row, int col) {
+    int i1 = (row + 1) % 3, i2 = (row + 2) % 3, j1 = (col + 1) % 3, j2 = (col + 2) % 3;
+    return A[j1][i1] * A[j2][i2] - A[j1][i2] * A[j2][i1];
+}
+
+static inline void matrixInverse(const double A[3][3], double R[3][3]) {
+    double det = A[0][0] * minorDet(A, 0, 0) + A[0][1] * minorDet(A, 1, 0) + A[0][2] * minorDet(A, 2, 0);
+    double T[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) T[r][c] = minorDet(A, r, c) / det;
+    double s = std::fabs(A[0][0]) + std::fabs(A[0][1]) + std::fabs(A[0][2]) + std::fabs(A[1][0]) +
+               std::fabs(A[1][1]) + std::fabs(A[1][2]) + std::fabs(A[2][0]) + std::fabs(A[2][1]) +
+               std::fabs(A[2][2]);
+    bool ok = std::fabs(det / s) > 1.e-6;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) R[r][c] = ok ? T[r][c] : 0.0;
+}
+
+struct FieldReq {
+    int density, velocity, velocity_gradient, velocity_std, scalar, scalar_gradient;
+};
+
+struct VertexData {
+    const double *rho;     // [nv]
+    const double *vel;     // [nv][3] or null
+    const double *scalar;  // [nv][S] or null
+    int S;
+};
+
+struct Accum {
+    double d;
+    double v[3];
+    double vg[9];
+    std::vector<double> s, sg;
+    explicit Accum(int S) : s(S), sg(3 * S) { clear(); }
+    void clear() {
+        d = 0;
+        for (double &x : v) x = 0;
+        for (double &x : vg) x = 0;
+        std::fill(s.begin(), s.end(), 0.0);
+        std::fill(sg.begin(), sg.end(), 0.0);
+    }
+};
+
+// Field values at p inside cell c, written with the reference's formulas
+// (positionMatrix, densityGrad/Value, velocityGrad/Value/Gradient,
+// scalarGrad/Value/Gradient; unaveraged_interpolation.cc:34-173).
+// value_bug=1 keeps the assignment-instead-of-accumulation of :116 / :159.
+void evaluate(const Mesh &m, const VertexData &vd, const FieldReq &f, int value_bug, int64_t c,
+              const double p[3], Accum &out, double *vel_value) {
+    const int32_t *cv = m.cells + 4 * c;
+    const double *x0 = P(m, cv[0]);
+    double A[3][3], Ai[3][3];
+    for (int v = 0; v < 3; ++v)
+        for (int i = 0; i < 3; ++i) A[v][i] = P(m, cv[v + 1])[i] - x0[i];
+    matrixInverse(A, Ai);
+    double rel[3] = {p[0] - x0[0], p[1] - x0[1], p[2] - x0[2]};
+    if (f.density) {
+        double dd[3], g[3];
+        for (int i = 0; i < 3; ++i) dd[i] = vd.rho[cv[i + 1]] - vd.rho[cv[0]];
+        for (int i = 0; i < 3; ++i) {
+            g[i] = 0.;
+            for (int j = 0; j < 3; ++j) g[i] += Ai[i][j] * dd[j];
+        }
+        double t = rel[0] * g[0] + rel[1] * g[1] + rel[2] * g[2];
+        out.d += vd.rho[cv[0]] + t;
+    }
+    if (f.velocity || f.velocity_gradient || f.velocity_std) {
+        const double *b = vd.vel + 3 * (int64_t)cv[0];
+        double T[3][3], G[3][3];
+        for (int v = 0; v < 3; ++v)
+            for (int k = 0; k < 3; ++k) T[v][k] = vd.vel[3 * (int64_t)cv[v + 1] + k] - b[k];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                G[i][j] = 0.;
+                for (int k = 0; k < 3; ++k) G[i][j] += Ai[i][k] * T[k][j];
+            }
+        double val[3];
+        for (int i = 0; i < 3; ++i) {
+            double t = 0.;
+            for (int j = 0; j < 3; ++j) {
+                if (value_bug) t = G[j][i] * rel[j];
+                else t += G[j][i] * rel[j];
+            }
+            val[i] = t + b[i];
+        }
+        if (f.velocity)
+            for (int i = 0; i < 3; ++i) out.v[i] += val[i];
+        if (f.velocity_gradient)
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) out.vg[j * 3 + i] += G[i][j];
+        if (vel_value)
+            for (int i = 0; i < 3; ++i) vel_value[i] = val[i];
+    }
+    if (f.scalar || f.scalar_gradient) {
+        const int S = vd.S;
+        const double *b = vd.scalar + (int64_t)S * cv[0];
+        std::vector<double> T(3 * S), G(3 * S);
+        for (int v = 0; v < 3; ++v)
+            for (int k = 0; k < S; ++k) T[v * S + k] = vd.scalar[(int64_t)S * cv[v + 1] + k] - b[k];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < S; ++j) {
+                double acc = 0.;
+                for (int k = 0; k < 3; ++k) acc += Ai[i][k] * T[k * S + j];
+                G[i * S + j] = acc;
+            }
+        for (int i = 0; i < S; ++i) {
+            double t = 0.;
+            for (int j = 0; j < 3; ++j) {
+                if (value_bug) t = G[j * S + i] * rel[j];
+                else t += G[j * S + i] * rel[j];
+            }
+            if (f.scalar) out.s[i] += t + b[i];
+        }
+        if (f.scalar_gradient)
+            for (int j = 0; j < S; ++j)
+                for (int i = 0; i < 3; ++i) out.sg[j * 3 + i] += G[i * S + j];
+    }
+}
+
+struct Outputs {
+    double *density, *velocity, *velocity_gradient, *velocity_std, *scalar, *scalar_gradient;
+    int64_t *cell_of_sample;  // optional: located cell per (output, sample)
+};
+
+void store(const FieldReq &f, int S, const Accum &a, double inv, int64_t o, Outputs &out) {
+    if (f.density && out.density) out.density[o] = a.d * inv;
+    if (f.velocity && out.velocity)
+        for (int i = 0; i < 3; ++i) out.velocity[3 * o + i] = a.v[i] * inv;
+    if (f.velocity_gradient && out.velocity_gradient)
+        for (int i = 0; i < 9; ++i) out.velocity_gradient[9 * o + i] = a.vg[i] * inv;
+    if (f.scalar && out.scalar)
+        for (int i = 0; i < S; ++i) out.scalar[(int64_t)S * o + i] = a.s[i] * inv;
+    if (f.scalar_gradient && out.scalar_gradient)
+        for (int i = 0; i < 3 * S; ++i) out.scalar_gradient[(int64_t)3 * S * o + i] = a.sg[i] * inv;
+}
+
+// Reference 'Real' division by NN: tempD/NN with NN an int.
+void store_avg(const FieldReq &f, int S, const Accum &a, int NN, int64_t o, Outputs &out) {
+    if (f.density && out.density) out.density[o] = a.d / NN;
+    if (f.velocity && out.velocity)
+        for (int i = 0; i < 3; ++i) out.velocity[3 * o + i] = a.v[i] / NN;
+    if (f.velocity_gradient && out.velocity_gradient)
+        for (int i = 0; i < 9; ++i) out.velocity_gradient[9 * o + i] = a.vg[i] / NN;
+    if (f.scalar && out.scalar)
+        for (int i = 0; i < S; ++i) out.scalar[(int64_t)S * o + i] = a.s[i] / NN;
+    if (f.scalar_gradient && out.scalar_gradient)
+        for (int i = 0; i < 3 * S; ++i) out.scalar_gradient[(int64_t)3 * S * o + i] = a.sg[i] / NN;
+}
+
+double stddev3(const std::vector<double> &v, int NN) {
+    double mean[3] = {0, 0, 0};
+    for (int i = 0; i < NN; ++i)
+        for (int j = 0; j < 3; ++j) mean[j] += v[3 * i + j];
+    for (int j = 0; j < 3; ++j) mean[j] /= NN;
+    double r = 0.;
+    for (int i = 0; i < NN; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double t = v[3 * i + j] - mean[j];
+            r += t * t;
+        }
+    return std::sqrt(r / NN);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- geometry pins ----------------------------------------------------------
+int oracle_orient_sos(const double *a, const double *b, const double *c, const double *p) {
+    double v;
+    return orient_sos(a, b, c, p, &v);
+}
+
+int oracle_orient_exact(const double *a, const double *b, const double *c, const double *p) {
+    return oracle_exact::orient3d_exact(a, b, c, p);
+}
+
+void oracle_mt19937(uint64_t seed, int64_t n, uint32_t *out) {
+    MT19937 g((uint32_t)seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = g.next();
+}
+
+void oracle_uniform(uint64_t seed, int64_t n, double *out) {
+    Uniform u(seed);
+    for (int64_t i = 0; i < n; ++i) out[i] = u();
+}
+
+// ---- vertex density (triangulation.cpp:200-279) ------------------------------
+// W_v = sum of finite incident cell volumes; rho_v = w_v * (4/rho_bar) / W_v;
+// 0 for vertices that touch an infinite cell (a hull face).
+void oracle_vertex_density(int64_t nv, const double *pos, int64_t nc, const int32_t *cells,
+                           const int32_t *nbrs, const double *weight, double rho_bar, double *rho,
+                           double *vol_sum, uint8_t *on_hull) {
+    Mesh m{nv, pos, nc, cells, nbrs};
+    std::vector<double> W(nv, 0.0);
+    std::vector<uint8_t> hull(nv, 0);
+    if (nv < 4) {
+        // fewer than 4 vertices: the reference warns and assigns nothing
+        for (int64_t v = 0; v < nv; ++v) rho[v] = 0.0;
+        return;
+    }
+    for (int64_t t = 0; t < nc; ++t) {
+        const int32_t *c = cells + 4 * t;
+        const double *p0 = P(m, c[0]), *p1 = P(m, c[1]), *p2 = P(m, c[2]), *p3 = P(m, c[3]);
+        double a[3], b[3], d[3];
+        for (int i = 0; i < 3; ++i) {
+            a[i] = p1[i] - p0[i];
+            b[i] = p2[i] - p0[i];
+            d[i] = p3[i] - p0[i];
+        }
+        double det = a[0] * (b[1] * d[2] - b[2] * d[1]) - a[1] * (b[0] * d[2] - b[2] * d[0]) +
+                     a[2] * (b[0] * d[1] - b[1] * d[0]);
+        double vol = det / 6.0;
+        for (int i = 0; i < 4; ++i) W[c[i]] += vol;
+        for (int i = 0; i < 4; ++i)
+            if (nbrs[4 * t + i] < 0)
+                for (int k = 0; k < 3; ++k) hull[c[FACE[i][k]]] = 1;
+    }
+    double factor = (3 + 1.) / rho_bar;
+    for (int64_t v = 0; v < nv; ++v) {
+        rho[v] = hull[v] ? 0.0 : weight[v] * factor / W[v];
+        if (vol_sum) vol_sum[v] = W[v];
+        if (on_hull) on_hull[v] = hull[v];
+    }
+}
+
+// ---- point location -----------------------------------------------------------
+// Locates each point, starting from the previous answer (as the reference's
+// loops pass the previous cell as hint). Returns the number of failed walks.
+int64_t oracle_locate(int64_t nv, const double *pos, int64_t nc, const int32_t *cells,
+                      const int32_t *nbrs, int64_t npts, const double *pts, int64_t *out_cell) {
+    Mesh m{nv, pos, nc, cells, nbrs};
+    int64_t hint = 0, failed = 0;
+    for (int64_t i = 0; i < npts; ++i) {
+        int64_t t = locate(m, pts + 3 * i, hint, nullptr);
+        if (t == -2) ++failed;
+        out_cell[i] = t;
+        if (t >= 0) hint = t;
+    }
+    return failed;
+}
+
+// ---- sampling drivers -------------------------------------------------------------
+// kind: 0 = regular grid (region box, n[3]); 1 = redshift cone (cone[6] in
+// degrees, origin[3], n[3]); 2 = user points (pts[npts][3], delta[npts][3],
+// region box). method: 0 = value at the sample point (uField), 2 = Monte
+// Carlo average, 3 = equidistant average (grid only). arg_order: order in
+// which the three draws of one grid sample are evaluated (0 = x,y,z as
+// clang does; 1 = z,y,x). Returns the number of output rows (user points
+// outside the region produce no row), or -1 if a walk failed.
+int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *cells,
+                      const int32_t *nbrs, const double *rho, const double *vel, const double *scalar,
+                      int S, int kind, const double *box, const int64_t *n, const double *cone,
+                      const double *origin, int64_t npts, const double *pts, const double *delta,
+                      int method, int NN, uint64_t seed, int partNo, int arg_order, int value_bug,
+                      const int *fields, double *o_density, double *o_velocity, double *o_velgrad,
+                      double *o_velstd, double *o_scalar, double *o_scalargrad,
+                      int64_t *o_cell_of_centre) {
+    Mesh m{nv, pos, nc, cells, nbrs};
+    VertexData vd{rho, vel, scalar, S};
+    FieldReq f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
+    Outputs out{o_density, o_velocity, o_velgrad, o_velstd, o_scalar, o_scalargrad, nullptr};
+    Accum acc(S > 0 ? S : 1);
+    if (method == 0) NN = 1;
+    std::vector<double> velsamples(3 * (size_t)(NN > 0 ? NN : 1));
+    uint64_t s = seed;
+    if (partNo >= 0) s += 1000ull * (uint64_t)partNo;
+    Uniform uni(s);
+    int64_t hint = 0;
+    int64_t prevc = 0;
+    bool failed = false;
+
+    auto loc = [&](const double p[3], int64_t h) -> int64_t {
+        int64_t t = locate(m, p, h < 0 ? prevc : h, nullptr);
+        if (t == -2) failed = true;
+        if (t >= 0) prevc = t;
+        return t;
+    };
+
+    if (kind == 0 || kind == 1) {
+        double bx[6];
+        for (int i = 0; i < 6; ++i) bx[i] = (kind == 0) ? box[i] : cone[i];
+        if (kind == 1) {
+            // checkAngles (triangulation_miscellaneous.cc:162-199), NO_DIM = 3
+            double *ang = bx + 2;
+            ang[2] -= std::floor(ang[2] / 360.) * 360.;
+            ang[3] -= std::floor(ang[3] / 360.) * 360.;
+            if (ang[2] > ang[3]) ang[3] += 360.;
+            const double deg = 3.14159265 / 180.;
+            for (int i = 0; i < 4; ++i) ang[i] *= deg;
+        }
+        double dx[3];
+        for (int i = 0; i < 3; ++i) dx[i] = (bx[2 * i + 1] - bx[2 * i]) / (double)n[i];
+        std::vector<std::array<double, 3>> sp(method == 3 ? (size_t)NN : 0);
+        int n3 = 0;
+        if (method == 3) {
+            n3 = (int)std::lround(std::cbrt((double)NN));
+            if (n3 * n3 * n3 != NN) return -1;
+            for (int i1 = 0; i1 < n3; ++i1)
+                for (int i2 = 0; i2 < n3; ++i2)
+                    for (int i3 = 0; i3 < n3; ++i3) {
+                        int idx = i1 * n3 * n3 + i2 * n3 + i3;
+                        sp[idx][2] = (i3 + 0.5) * dx[2] / n3 - dx[2] / 2.;
+                        sp[idx][0] = (i1 + 0.5) * dx[0] / n3 - dx[0] / 2.;
+                        sp[idx][1] = (i2 + 0.5) * dx[1] / n3 - dx[1] / 2.;
+                    }
+        }
+        int64_t o = 0;
+        double x = bx[0] + dx[0] / 2.;
+        for (int64_t i = 0; i < n[0]; ++i) {
+            double y = bx[2] + dx[1] / 2.;
+            for (int64_t j = 0; j < n[1]; ++j) {
+                double z = bx[4] + dx[2] / 2.;
+                for (int64_t k = 0; k < n[2]; ++k, ++o) {
+                    double c[3];
+                    if (kind == 0) {
+                        c[0] = x; c[1] = y; c[2] = z;
+                    } else {
+                        c[0] = origin[0] + x * std::sin(y) * std::cos(z);
+                        c[1] = origin[1] + x * std::sin(y) * std::sin(z);
+                        c[2] = origin[2] + x * std::cos(y);
+                    }
+                    int64_t centre = loc(c, hint);
+                    if (o_cell_of_centre) o_cell_of_centre[o] = centre;
+                    if (centre >= 0) hint = centre;
+                    acc.clear();
+                    if (method == 0) {
+                        if (centre >= 0) evaluate(m, vd, f, value_bug, centre, c, acc, nullptr);
+                        store(f, S, acc, 1.0, o, out);
+                        z += dx[2];
+                        continue;
+                    }
+                    for (int w = 0; w < NN; ++w) {
+                        double p[3];
+                        if (method == 3) {
+                            p[0] = x + sp[w][0]; p[1] = y + sp[w][1]; p[2] = z + sp[w][2];
+                        } else if (kind == 0) {
+                            double u0, u1, u2;
+                            if (arg_order == 0) { u0 = uni(); u1 = uni(); u2 = uni(); }
+                            else { u2 = uni(); u1 = uni(); u0 = uni(); }
+                            p[0] = u0 * dx[0] + x; p[1] = u1 * dx[1] + y; p[2] = u2 * dx[2] + z;
+                        } else {
+                            double tr = x + uni() * dx[0];
+                            double tt = y + uni() * dx[1];
+                            double tp = z + uni() * dx[2];
+                            p[0] = origin[0] + tr * std::sin(tt) * std::cos(tp);
+                            p[1] = origin[1] + tr * std::sin(tt) * std::sin(tp);
+                            p[2] = origin[2] + tr * std::cos(tt);
+                        }
+                        int64_t t = loc(p, centre >= 0 ? centre : -1);
+                        double *vv = &velsamples[3 * w];
+                        vv[0] = vv[1] = vv[2] = 0.;
+                        if (t < 0) continue;
+                        evaluate(m, vd, f, value_bug, t, p, acc,
+                                 (kind == 0 && method == 2 && f.velocity_std) ? vv : nullptr);
+                    }
+                    store_avg(f, S, acc, NN, o, out);
+                    if (kind == 0 && method == 2 && f.velocity_std && out.velocity_std)
+                        out.velocity_std[o] = stddev3(velsamples, NN);
+                    z += dx[2];
+                }
+                y += dx[1];
+            }
+            x += dx[0];
+        }
+        return failed ? -1 : o;
+    }
+
+    // user points (unaveraged_interpolation.cc:560-691, averaged_interpolation_2.cc:470-647)
+    int64_t o = 0;
+    for (int64_t i = 0; i < npts; ++i) {
+        const double *q = pts + 3 * i;
+        if (!(q[0] >= box[0] && q[0] <= box[1] && q[1] >= box[2] && q[1] <= box[3] && q[2] >= box[4] &&
+              q[2] <= box[5]))
+            continue;
+        double c[3] = {q[0], q[1], q[2]};
+        int64_t centre = loc(c, hint);
+        if (o_cell_of_centre) o_cell_of_centre[o] = centre;
+        if (centre >= 0) hint = centre;
+        acc.clear();
+        if (method == 0) {
+            if (centre >= 0) evaluate(m, vd, f, value_bug, centre, c, acc, nullptr);
+            store(f, S, acc, 1.0, o, out);
+            ++o;
+            continue;
+        }
+        const double *dl = delta + 3 * i;
+        for (int w = 0; w < NN; ++w) {
+            double p[3];
+            p[0] = q[0] + uni() * dl[0];
+            p[1] = q[1] + uni() * dl[1];
+            p[2] = q[2] + uni() * dl[2];
+            int64_t t = loc(p, centre >= 0 ? centre : -1);
+            if (t < 0) continue;
+            evaluate(m, vd, f, value_bug, t, p, acc, nullptr);
+        }
+        store_avg(f, S, acc, NN, o, out);
+        ++o;
+    }
+    return failed ? -1 : o;
+}
+
+// ---- derived velocity fields (DTFE.cpp:394-433) -------------------------------------
+void oracle_div_shear_vort(int64_t n, const double *grad, double *div, double *shear, double *vort) {
+    for (int64_t c = 0; c < n; ++c) {
+        const double *g = grad + 9 * c;
+        if (div) div[c] = g[0] + g[4] + g[8];
+        if (shear) {
+            double t[5];
+            int idx = 0;
+            for (int i = 0; i < 2; ++i)
+                for (int j = i; j < 3; ++j) t[idx++] = (g[i * 3 + j] + g[j * 3 + i]) / 2.;
+            double tr = (t[0] + t[3] + g[8]) / 3.;
+            t[0] -= tr;
+            t[3] -= tr;
+            for (int i = 0; i < 5; ++i) shear[5 * c + i] = t[i];
+        }
+        if (vort) {
+            int idx = 0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = i + 1; j < 3; ++j) vort[3 * c + idx++] = (g[i * 3 + j] - g[j * 3 + i]) / 2.;
+        }
+    }
+}
+
+// ---- CIC / TSC deposits --------------------------------------------------------------
+// scheme: 0 = CIC (CIC_interpolation.cc:67-273), 1 = TSC (TSC_interpolation.cc:67-254).
+// Outputs: density (normalised by N_cells / 35000^3 / rho_bar) and velocity
+// (momentum / mass). want_density / want_velocity follow aField.
+void oracle_deposit(int scheme, int64_t np, const double *pos, const double *weight, const double *vel,
+                    const double *box, const int64_t *n, double rho_bar, int want_density,
+                    int want_velocity, double *density, double *velocity) {
+    const int64_t ncell = n[0] * n[1] * n[2];
+    std::vector<double> mass(ncell, 0.0), mom(3 * ncell, 0.0);
+    double dx[3], outer[6], inner[6];
+    for (int i = 0; i < 3; ++i) dx[i] = (box[2 * i + 1] - box[2 * i]) / (double)n[i];
+    for (int i = 0; i < 3; ++i) {
+        outer[2 * i] = box[2 * i] - dx[i];
+        outer[2 * i + 1] = box[2 * i + 1] + dx[i];
+        inner[2 * i] = box[2 * i] - (-1.1 * dx[i]);
+        inner[2 * i + 1] = box[2 * i + 1] + (-1.1 * dx[i]);
+    }
+    auto inBox = [](const double *b, const double *p) {
+        return p[0] >= b[0] && p[0] <= b[1] && p[1] >= b[2] && p[1] <= b[3] && p[2] >= b[4] && p[2] <= b[5];
+    };
+    auto deposit = [&](int64_t pi, int cnt[3], int cell[3][3], double wgt[3][3]) {
+        for (int i1 = 0; i1 < cnt[0]; ++i1)
+            for (int i2 = 0; i2 < cnt[1]; ++i2)
+                for (int i3 = 0; i3 < cnt[2]; ++i3) {
+                    int64_t idx = (int64_t)cell[0][i1] * n[1] * n[2] + (int64_t)cell[1][i2] * n[2] + cell[2][i3];
+                    double r = weight[pi] * wgt[0][i1] * wgt[1][i2] * wgt[2][i3];
+                    mass[idx] += r;
+                    for (int k = 0; k < 3; ++k) mom[3 * idx + k] += vel[3 * pi + k] * r;
+                }
+    };
+    // inner particles first, then the outer ones (two passes, as the reference)
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t pi = 0; pi < np; ++pi) {
+            const double *p = pos + 3 * pi;
+            bool in = inBox(inner, p);
+            bool out = !in && inBox(outer, p);
+            if ((pass == 0 && !in) || (pass == 1 && !out)) continue;
+            int cell[3][3], cnt[3];
+            double wgt[3][3];
+            for (int j = 0; j < 3; ++j) {
+                double t = (p[j] - box[2 * j]) / dx[j];
+                if (pass == 0) {
+                    cell[j][0] = (int)std::floor(t) - 1;
+                    cell[j][1] = cell[j][0] + 1;
+                    cell[j][2] = cell[j][1] + 1;
+                    t -= cell[j][1] + 0.5;
+                    cnt[j] = 3;
+                    if (scheme == 0) {
+                        if (t < 0.) { wgt[j][0] = -t; wgt[j][1] = 1 + t; wgt[j][2] = 0.; }
+                        else { wgt[j][0] = 0.; wgt[j][1] = 1 - t; wgt[j][2] = t; }
+                    } else {
+                        wgt[j][0] = .5 * (0.5 - t) * (0.5 - t);
+                        wgt[j][1] = .75 - t * t;
+                        wgt[j][2] = .5 * (0.5 + t) * (0.5 + t);
+                    }
+                } else {
+                    int ti = (int)std::floor(t);
+                    t -= ti + 0.5;
+                    int ng = (int)n[j];
+                    cnt[j] = 0;
+                    if (ti == -1) {
+                        cell[j][0] = 0; cnt[j] = 1;
+                        wgt[j][0] = scheme == 0 ? t : .5 * (0.5 + t) * (0.5 + t);
+                    } else if (ti == 0) {
+                        cell[j][0] = 0; cell[j][1] = 1; cnt[j] = 2;
+                        if (scheme == 0) { wgt[j][0] = 1 - t; wgt[j][1] = t; }
+                        else { wgt[j][0] = .75 - t * t; wgt[j][1] = .5 * (0.5 + t) * (0.5 + t); }
+                    } else if (ti == ng - 1) {
+                        cell[j][0] = ng - 2; cell[j][1] = ng - 1; cnt[j] = 2;
+                        if (scheme == 0) { wgt[j][0] = -t; wgt[j][1] = 1 + t; }
+                        else { wgt[j][0] = .5 * (0.5 - t) * (0.5 - t); wgt[j][1] = .75 - t * t; }
+                    } else if (ti == ng) {
+                        cell[j][0] = ng - 1; cnt[j] = 1;
+                        wgt[j][0] = scheme == 0 ? -t : .5 * (0.5 - t) * (0.5 - t);
+                    } else if (!(ti < -1 || ti > ng)) {
+                        cell[j][0] = ti - 1; cell[j][1] = ti; cell[j][2] = ti + 1; cnt[j] = 3;
+                        if (scheme == 0) {
+                            if (t < 0.) { wgt[j][0] = -t; wgt[j][1] = 1 + t; wgt[j][2] = 0.; }
+                            else { wgt[j][0] = 0.; wgt[j][1] = 1 - t; wgt[j][2] = t; }
+                        } else {
+                            wgt[j][0] = .5 * (0.5 - t) * (0.5 - t);
+                            wgt[j][1] = .75 - t * t;
+                            wgt[j][2] = .5 * (0.5 + t) * (0.5 + t);
+                        }
+                    }
+                }
+            }
+            deposit(pi, cnt, cell, wgt);
+        }
+    }
+    if (want_velocity && velocity)
+        for (int64_t i = 0; i < ncell; ++i)
+            for (int k = 0; k < 3; ++k) velocity[3 * i + k] = mass[i] != 0.0 ? mom[3 * i + k] / mass[i] : 0.0;
+    if (want_density && density) {
+        double factor = (double)ncell / (35000. * 35000. * 35000.) / rho_bar;
+        for (int64_t i = 0; i < ncell; ++i) density[i] = mass[i] * factor;
+    }
+}
+
+// ---- host front end: periodic padding and average density -----------------------------
+// findParticlesInBox / insertParticlesInBox (subpartition.h:79-154): every
+// translate by (i-1,j-1,k-1)*L of the padded box is tested (the overlap test
+// at :129 accepts all 27), and particles inside the translated box (closed
+// bounds) are copied with pos -= offset. Returns the image count; fills
+// image_of (original index) when given, in the reference's output order.
+int64_t oracle_periodic_images(int64_t np, const double *pos, const double *full_box,
+                               const double *padded_box, int64_t cap, int64_t *image_of,
+                               double *image_pos) {
+    double L[3];
+    for (int i = 0; i < 3; ++i) L[i] = full_box[2 * i + 1] - full_box[2 * i];
+    int64_t k = 0;
+    for (int i1 = 0; i1 < 3; ++i1)
+        for (int i2 = 0; i2 < 3; ++i2)
+            for (int i3 = 0; i3 < 3; ++i3) {
+                double off[3] = {(i1 - 1) * L[0], (i2 - 1) * L[1], (i3 - 1) * L[2]};
+                double b[6];
+                for (int i = 0; i < 3; ++i) {
+                    b[2 * i] = padded_box[2 * i] + off[i];
+                    b[2 * i + 1] = padded_box[2 * i + 1] + off[i];
+                }
+                for (int64_t p = 0; p < np; ++p) {
+                    const double *q = pos + 3 * p;
+                    if (q[0] >= b[0] && q[0] <= b[1] && q[1] >= b[2] && q[1] <= b[3] && q[2] >= b[4] &&
+                        q[2] <= b[5]) {
+                        if (k < cap) {
+                            if (image_of) image_of[k] = p;
+                            if (image_pos)
+                                for (int i = 0; i < 3; ++i) image_pos[3 * k + i] = q[i] - off[i];
+                        }
+                        ++k;
+                    }
+                }
+            }
+    return k;
+}
+
+double oracle_average_density(int64_t np, const double *weight) {
+    double total = 0.;
+    for (int64_t i = 0; i < np; ++i) total += weight[i];
+    return total / (35000. * 35000. * 35000.);
+}
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+"""TEST INFRASTRUCTURE — Python binding of the CPU FP64 oracle.
+
+Only tests/, the smoke check and the CPU-baseline leg of the benchmark use
+this module. It loads oracle/_build/libdtfe_oracle.so (built by
+oracle/Makefile) and exposes numpy-level functions; see dtfe_oracle.cpp for
+the reference lines each one restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libdtfe_oracle.so")
+
+FIELDS = ["density", "velocity", "velocity_gradient", "velocity_std", "scalar", "scalar_gradient"]
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    P = C.c_void_p
+    i64, i32, d = C.c_int64, C.c_int, C.c_double
+    lib.oracle_orient_sos.argtypes = [P, P, P, P]
+    lib.oracle_orient_exact.argtypes = [P, P, P, P]
+    lib.oracle_mt19937.argtypes = [C.c_uint64, i64, P]
+    lib.oracle_uniform.argtypes = [C.c_uint64, i64, P]
+    lib.oracle_vertex_density.argtypes = [i64, P, i64, P, P, P, d, P, P, P]
+    lib.oracle_locate.argtypes = [i64, P, i64, P, P, i64, P, P]
+    lib.oracle_locate.restype = i64
+    lib.oracle_sample.argtypes = [i64, P, i64, P, P, P, P, P, i32, i32, P, P, P, P, i64, P, P,
+                                  i32, i32, C.c_uint64, i32, i32, i32, P, P, P, P, P, P, P, P]
+    lib.oracle_sample.restype = i64
+    lib.oracle_div_shear_vort.argtypes = [i64, P, P, P, P]
+    lib.oracle_deposit.argtypes = [i32, i64, P, P, P, P, P, d, i32, i32, P, P]
+    lib.oracle_periodic_images.argtypes = [i64, P, P, P, i64, P, P]
+    lib.oracle_periodic_images.restype = i64
+    lib.oracle_average_density.argtypes = [i64, P]
+    lib.oracle_average_density.restype = d
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def orient_sos(a, b, c, p) -> int:
+    lib = load()
+    a, b, c, p = (_f64(x) for x in (a, b, c, p))
+    return int(lib.oracle_orient_sos(_p(a), _p(b), _p(c), _p(p)))
+
+
+def orient_exact(a, b, c, p) -> int:
+    lib = load()
+    a, b, c, p = (_f64(x) for x in (a, b, c, p))
+    return int(lib.oracle_orient_exact(_p(a), _p(b), _p(c), _p(p)))
+
+
+def mt19937(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint32)
+    load().oracle_mt19937(seed, n, _p(out))
+    return out
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    load().oracle_uniform(seed, n, _p(out))
+    return out
+
+
+def vertex_density(mesh, weight, rho_bar):
+    lib = load()
+    pos, cells, nbrs = _f64(mesh.pos), _i32(mesh.cells), _i32(mesh.nbrs)
+    w = _f64(weight)
+    nv = pos.shape[0]
+    rho = np.zeros(nv)
+    W = np.zeros(nv)
+    hull = np.zeros(nv, dtype=np.uint8)
+    lib.oracle_vertex_density(nv, _p(pos), cells.shape[0], _p(cells), _p(nbrs), _p(w), rho_bar, _p(rho),
+                              _p(W), _p(hull))
+    return rho, W, hull.astype(bool)
+
+
+def locate(mesh, pts):
+    lib = load()
+    pos, cells, nbrs = _f64(mesh.pos), _i32(mesh.cells), _i32(mesh.nbrs)
+    pts = _f64(pts)
+    out = np.empty(pts.shape[0], dtype=np.int64)
+    failed = lib.oracle_locate(pos.shape[0], _p(pos), cells.shape[0], _p(cells), _p(nbrs), pts.shape[0],
+                               _p(pts), _p(out))
+    if failed:
+        raise RuntimeError(f"oracle walk failed for {failed} points")
+    return out
+
+
+def sample(mesh, rho, vel=None, scalar=None, fields=("density",), kind="grid", box=None, n=None,
+           cone=None, origin=None, points=None, deltas=None, method=0, samples=1, seed=0, part_no=-1,
+           arg_order=0, value_bug=True):
+    """Runs the oracle's sampling driver; returns a dict of float64 arrays."""
+    lib = load()
+    pos, cells, nbrs = _f64(mesh.pos), _i32(mesh.cells), _i32(mesh.nbrs)
+    rho, vel = _f64(rho), _f64(vel)
+    scalar = _f64(scalar)
+    S = 0 if scalar is None else scalar.shape[1]
+    kinds = {"grid": 0, "cone": 1, "points": 2}
+    k = kinds[kind]
+    if k == 2:
+        pts = _f64(points)
+        npts = pts.shape[0]
+        n_rows = npts
+        dl = _f64(deltas) if deltas is not None else np.zeros_like(pts)
+        nn = np.ones(3, dtype=np.int64)
+    else:
+        pts, dl, npts = None, None, 0
+        nn = np.asarray(n, dtype=np.int64)
+        n_rows = int(np.prod(nn))
+    bx = _f64(box if box is not None else np.zeros(6))
+    cn = _f64(cone if cone is not None else np.zeros(6))
+    og = _f64(origin if origin is not None else np.zeros(3))
+    want = {f: (f in fields) for f in FIELDS}
+    flags = np.array([int(want[f]) for f in FIELDS], dtype=np.int32)
+    outs = {}
+    comp = {"density": 1, "velocity": 3, "velocity_gradient": 9, "velocity_std": 1, "scalar": S,
+            "scalar_gradient": 3 * S}
+    for f in FIELDS:
+        outs[f] = np.zeros((n_rows, comp[f]) if comp[f] != 1 else n_rows) if want[f] else None
+    cells_out = np.zeros(n_rows, dtype=np.int64)
+    nw = lib.oracle_sample(pos.shape[0], _p(pos), cells.shape[0], _p(cells), _p(nbrs), _p(rho), _p(vel),
+                           _p(scalar), S, k, _p(bx), _p(nn), _p(cn), _p(og), npts, _p(pts), _p(dl),
+                           method, samples, seed, part_no, arg_order, int(value_bug), _p(flags),
+                           *[_p(outs[f]) for f in FIELDS], _p(cells_out))
+    if nw < 0:
+        raise RuntimeError("oracle sampling failed (walk did not terminate or bad arguments)")
+    res = {f: outs[f][:nw] for f in FIELDS if want[f]}
+    res["located_cell"] = cells_out[:nw]
+    grad_needed = [f for f in ("velocity_divergence", "velocity_shear", "velocity_vorticity") if f in fields]
+    if grad_needed:
+        if "velocity_gradient" not in res:
+            raise ValueError("derived velocity fields need velocity_gradient in the oracle call")
+        g = np.ascontiguousarray(res["velocity_gradient"])
+        dv = np.zeros(nw)
+        sh = np.zeros((nw, 5))
+        vo = np.zeros((nw, 3))
+        lib.oracle_div_shear_vort(nw, _p(g), _p(dv), _p(sh), _p(vo))
+        if "velocity_divergence" in fields:
+            res["velocity_divergence"] = dv
+        if "velocity_shear" in fields:
+            res["velocity_shear"] = sh
+        if "velocity_vorticity" in fields:
+            res["velocity_vorticity"] = vo
+    return res
+
+
+def div_shear_vort(grad):
+    lib = load()
+    g = _f64(grad).reshape(-1, 9)
+    n = g.shape[0]
+    dv, sh, vo = np.zeros(n), np.zeros((n, 5)), np.zeros((n, 3))
+    lib.oracle_div_shear_vort(n, _p(g), _p(dv), _p(sh), _p(vo))
+    return dv, sh, vo
+
+
+def deposit(scheme, pos, weight, vel, box, n, rho_bar, want_density=True, want_velocity=True):
+    lib = load()
+    pos, weight = _f64(pos), _f64(weight)
+    vel = _f64(vel) if vel is not None else np.zeros_like(pos)
+    nn = np.asarray(n, dtype=np.int64)
+    ncell = int(np.prod(nn))
+    dens = np.zeros(ncell)
+    v = np.zeros((ncell, 3))
+    lib.oracle_deposit(0 if scheme == "cic" else 1, pos.shape[0], _p(pos), _p(weight), _p(vel), _p(_f64(box)),
+                       _p(nn), rho_bar, int(want_density), int(want_velocity), _p(dens), _p(v))
+    return dens, v
+
+
+def periodic_images(pos, full_box, padded_box):
+    lib = load()
+    pos = _f64(pos)
+    fb, pb = _f64(full_box), _f64(padded_box)
+    n = lib.oracle_periodic_images(pos.shape[0], _p(pos), _p(fb), _p(pb), 0, None, None)
+    iof = np.empty(n, dtype=np.int64)
+    ip = np.empty((n, 3))
+    lib.oracle_periodic_images(pos.shape[0], _p(pos), _p(fb), _p(pb), n, _p(iof), _p(ip))
+    return ip, iof
+
+
+def average_density(weight):
+    w = _f64(weight)
+    return float(load().oracle_average_density(w.shape[0], _p(w)))

@@ -1,0 +1,236 @@
+"""GPU parity: libdtfe_hip.so against the CPU FP64 oracle on the same meshes.
+
+Tolerances (relative to the largest magnitude of the oracle field):
+  * all-FP64 path: 1e-10 (only summation order and reciprocal-vs-division
+    rounding differ from the oracle);
+  * mixed path (FP64 geometry, FP32 fields): 1e-5 at cell centres and 1e-4
+    for cell averages, the bar of the mixed-precision target;
+  * located tetrahedra: bit-exact (both sides use exact signs with the same
+    symbolic tie-break).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32_CENTRE = 1e-5
+TOL32_AVG = 1e-4
+
+
+def rel_err(got, exp):
+    got = np.asarray(got, dtype=np.float64)
+    exp = np.asarray(exp, dtype=np.float64)
+    scale = max(np.max(np.abs(exp)), 1e-300)
+    return float(np.max(np.abs(got - exp)) / scale)
+
+
+def gpu_setup(prob, dtype):
+    from dtfe_amd import api
+    dm = api.DeviceMesh.from_mesh(prob.mesh, "cuda:0")
+    w = torch.from_numpy(prob.w).to("cuda:0", dtype)
+    rho = api.vertex_density(dm, w, prob.rho_bar)
+    vel = torch.from_numpy(prob.vel).to("cuda:0", dtype) if prob.vel is not None else None
+    sc = torch.from_numpy(prob.scalar).to("cuda:0", dtype) if prob.scalar is not None else None
+    return dm, rho, vel, sc
+
+
+def oracle_rho(prob):
+    rho, _, _ = O.vertex_density(prob.mesh, prob.w, prob.rho_bar)
+    return rho
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-6)])
+def test_vertex_density(gpu, dtype, tol):
+    prob = P.clustered_problem()
+    dm, rho, _, _ = gpu_setup(prob, dtype)
+    exp = oracle_rho(prob)
+    got = rho.cpu().numpy().astype(np.float64)
+    assert np.array_equal(got == 0, exp == 0), "hull vertices differ"
+    nz = exp != 0
+    err = np.max(np.abs(got[nz] - exp[nz]) / exp[nz])
+    print(f"vertex density {dtype}: max rel err {err:.3e}")
+    assert err < tol
+
+
+CENTRE_FIELDS = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_shear",
+                 "velocity_vorticity", "scalar", "scalar_gradient"]
+
+
+@pytest.mark.parametrize("which", ["uniform", "clustered"])
+@pytest.mark.parametrize("value_bug", [True, False])
+def test_grid_centre_fp64(gpu, which, value_bug):
+    from dtfe_amd import api
+    prob = P.uniform_problem() if which == "uniform" else P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float64)
+    n = (24, 20, 16)
+    out = api.sample(dm, rho, vel, sc, CENTRE_FIELDS, api.GridSpec(prob.region, n), method=0,
+                     value_bug=value_bug, want_cells=True)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, prob.scalar,
+                   fields=CENTRE_FIELDS, kind="grid", box=prob.region, n=n, method=0, value_bug=value_bug)
+    assert np.array_equal(out["located_cell"].cpu().numpy(), ref["located_cell"]), "located cells differ"
+    for f in CENTRE_FIELDS:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"centre fp64 {which} bug={value_bug} {f}: {e:.3e}")
+        assert e < TOL64, f
+
+
+@pytest.mark.parametrize("which", ["uniform", "clustered"])
+def test_grid_centre_mixed(gpu, which):
+    from dtfe_amd import api
+    prob = P.uniform_problem() if which == "uniform" else P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float32)
+    n = (24, 20, 16)
+    fields = ["density", "velocity", "velocity_gradient", "velocity_divergence"]
+    out = api.sample(dm, rho, vel, None, fields, api.GridSpec(prob.region, n), method=0, want_cells=True)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="grid", box=prob.region,
+                   n=n, method=0)
+    assert np.array_equal(out["located_cell"].cpu().numpy(), ref["located_cell"])
+    # density: per-cell relative error (the mixed path's target metric)
+    d_got = out["density"].cpu().numpy().astype(np.float64)
+    d_exp = ref["density"]
+    nz = d_exp > 0
+    e_cell = float(np.max(np.abs(d_got[nz] - d_exp[nz]) / d_exp[nz]))
+    print(f"centre mixed {which} density per-cell rel err {e_cell:.3e}")
+    assert e_cell < TOL32_CENTRE
+    for f in fields[1:]:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"centre mixed {which} {f}: {e:.3e}")
+        assert e < TOL32_CENTRE, f
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("arg_order", [0, 1])
+def test_grid_monte_carlo(gpu, dtype, arg_order):
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, dtype)
+    n = (20, 18, 17)
+    fields = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity",
+              "velocity_std", "scalar", "scalar_gradient"]
+    out = api.sample(dm, rho, vel, sc, fields, api.GridSpec(prob.region, n), method=2, samples=8, seed=12345,
+                     arg_order=arg_order)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, prob.scalar, fields=fields, kind="grid",
+                   box=prob.region, n=n, method=2, samples=8, seed=12345, arg_order=arg_order)
+    tol = TOL64 if dtype == torch.float64 else TOL32_AVG
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"MC {dtype} order={arg_order} {f}: {e:.3e}")
+        assert e < tol, f
+
+
+def test_grid_monte_carlo_many_blocks(gpu):
+    """Rows spread over many workgroups: MT plan states at every block start."""
+    from dtfe_amd import api
+    prob = P.uniform_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float64)
+    n = (64, 64, 40)
+    out = api.sample(dm, rho, vel, None, ["density", "velocity"], api.GridSpec(prob.region, n), method=2,
+                     samples=5, seed=99, part_no=2)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=("density", "velocity"), kind="grid",
+                   box=prob.region, n=n, method=2, samples=5, seed=99, part_no=2)
+    for f in ("density", "velocity"):
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"MC many blocks {f}: {e:.3e}")
+        assert e < TOL64
+
+
+def test_grid_equidistant(gpu):
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float64)
+    n = (12, 12, 12)
+    fields = ["density", "velocity", "velocity_gradient"]
+    out = api.sample(dm, rho, vel, None, fields, api.GridSpec(prob.region, n), method=3, samples=27)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="grid", box=prob.region,
+                   n=n, method=3, samples=27)
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"equidistant {f}: {e:.3e}")
+        assert e < TOL64
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_cone(gpu, method):
+    from dtfe_amd import api
+    prob = P.uniform_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float64)
+    cone = (1.0, 9.0, 15.0, 75.0, 15.0, 75.0)
+    n = (16, 8, 8)
+    fields = ["density", "velocity", "velocity_gradient"]
+    out = api.sample(dm, rho, vel, None, fields, api.ConeSpec(cone, n, (0.5, 0.5, 0.5)), method=method,
+                     samples=4, seed=5)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="cone", cone=cone, n=n,
+                   origin=(0.5, 0.5, 0.5), method=method, samples=4, seed=5)
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"cone method={method} {f}: {e:.3e}")
+        # sin/cos of the MC angles come from the device math library: ulp-level differences
+        assert e < (TOL64 if method == 0 else 1e-9)
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_user_points(gpu, method):
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float64)
+    rng = np.random.default_rng(21)
+    pts = rng.random((3000, 3)) * 24.0 - 2.0       # some outside the region box
+    deltas = np.full_like(pts, 0.5)
+    box = (0.0, 20.0, 0.0, 20.0, 0.0, 20.0)
+    fields = ["density", "velocity", "velocity_gradient"]
+    out = api.sample(dm, rho, vel, None, fields,
+                     api.PointsSpec(torch.from_numpy(pts).cuda(), torch.from_numpy(deltas).cuda(), box),
+                     method=method, samples=6, seed=77)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="points", box=box,
+                   points=pts, deltas=deltas, method=method, samples=6, seed=77)
+    assert out["density"].shape[0] == ref["density"].shape[0]
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"points method={method} {f}: {e:.3e}")
+        assert e < TOL64
+
+
+def test_vertex_ties(gpu):
+    """Samples exactly on mesh vertices: every face through the vertex is an
+    exact tie, resolved by exact signs and the symbolic tie-break."""
+    from dtfe_amd import api
+    prob = P.uniform_problem()
+    dm, rho, vel, _ = gpu_setup(prob, torch.float64)
+    rng = np.random.default_rng(5)
+    inside = np.nonzero(np.all((prob.mesh.pos > 0.5) & (prob.mesh.pos < 9.5), axis=1))[0]
+    pts = np.ascontiguousarray(prob.mesh.pos[rng.choice(inside, 400, replace=False)])
+    box = prob.region
+    out = api.sample(dm, rho, vel, None, ["density", "velocity_gradient"],
+                     api.PointsSpec(torch.from_numpy(pts).cuda(), None, box), method=0, want_cells=True)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=("density", "velocity_gradient"),
+                   kind="points", box=box, points=pts, method=0)
+    assert np.array_equal(out["located_cell"].cpu().numpy(), ref["located_cell"])
+    for f in ("density", "velocity_gradient"):
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"ties {f}: {e:.3e}")
+        assert e < TOL64
+
+
+@pytest.mark.parametrize("scheme", ["cic", "tsc"])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+def test_deposits(gpu, scheme, dtype, tol):
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    ps = prob.ps
+    box = ps.box
+    n = (16, 16, 16)
+    rho_bar = prob.rho_bar
+    out = api.deposit(scheme, torch.from_numpy(ps.pos).cuda(), torch.from_numpy(ps.weight).to("cuda", dtype),
+                      torch.from_numpy(ps.vel).to("cuda", dtype), box, n, rho_bar)
+    d, v = O.deposit(scheme, ps.pos, ps.weight, ps.vel, box, n, rho_bar)
+    e1 = rel_err(out["density"].cpu().numpy(), d)
+    e2 = rel_err(out["velocity"].cpu().numpy(), v)
+    print(f"{scheme} {dtype}: density {e1:.3e} velocity {e2:.3e}")
+    assert e1 < tol and e2 < (tol * 100 if dtype == torch.float32 else tol)
