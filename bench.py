@@ -1,0 +1,223 @@
+"""Benchmark of the DTFE interpolation path on one MI355X.
+
+Workload (one "step" = one pass of the post-tessellation path):
+  clustered particle set, N = 128^3, periodic box L = 200, 2000 Gaussian
+  blobs (sigma log-uniform in [0.5, 3]), lognormal(0, 0.3) weights,
+  velocities 100 (x - c_blob) + N(0, 300^2); periodic padding of 5
+  interparticle spacings; vertex densities, then density, velocity,
+  velocity gradient, divergence and vorticity averaged over each cell of a
+  256^3 grid with 8 Monte-Carlo samples per cell (the reference's averaging
+  method 2 and its mt19937 sample stream).
+
+Prints one JSON line: grid samples/s (value), the roofline of the dominant
+kernel (algorithmic HBM bytes / its HIP-event duration against 8 TB/s) and
+the CPU oracle timed on a bounded slab of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-side", type=int, default=128)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--samples", type=int, default=8)
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--tess", default="auto")
+    ap.add_argument("--mesh-cache", default="")
+    ap.add_argument("--cpu-slabs", type=int, default=2, help="x-slabs of the grid timed on the CPU oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
+    ap.add_argument("--seed", type=int, default=2)
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_mesh(args):
+    from dtfe_amd import mesh as M, synthetic as S
+    ps = S.clustered_velocity(seed=args.seed, n_side=args.n_side)
+    prep = M.prepare_particles(ps)
+    if args.mesh_cache and os.path.exists(args.mesh_cache):
+        z = np.load(args.mesh_cache)
+        m = M.Mesh(pos=z["pos"], cells=z["cells"], nbrs=z["nbrs"], point_of_vertex=z["pov"],
+                   vertex_of_point=z["vop"], backend=str(z["backend"]), seconds=float(z["seconds"]))
+        log(f"mesh loaded from cache: {m.n_vertices} vertices, {m.n_cells} cells")
+    else:
+        t0 = time.perf_counter()
+        m = M.tessellate(prep.pos, backend=args.tess)
+        log(f"tessellation ({m.backend}): {m.n_vertices} vertices, {m.n_cells} cells, "
+            f"{time.perf_counter() - t0:.1f} s")
+        if args.mesh_cache:
+            np.savez(args.mesh_cache, pos=m.pos, cells=m.cells, nbrs=m.nbrs, pov=m.point_of_vertex,
+                     vop=m.vertex_of_point, backend=m.backend, seconds=m.seconds)
+    return ps, prep, m
+
+
+FIELDS = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity"]
+OUT_FLOATS = 1 + 3 + 9 + 1 + 3
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from dtfe_amd import api, mesh as M
+
+    ps, prep, m = build_mesh(args)
+    dev = torch.device("cuda:0")
+    dt = torch.float32 if args.dtype == "f32" else torch.float64
+    w = torch.from_numpy(M.vertex_data(m, prep.weight)).to(dev, dt)
+    vel = torch.from_numpy(M.vertex_data(m, prep.vel)).to(dev, dt)
+    dm = api.DeviceMesh.from_mesh(m, dev)
+    box = prep.region
+    n = (args.grid,) * 3
+    spec = api.GridSpec(box, n)
+    mc_seed = 20240601
+
+    def step(dtype_vel, dtype_w):
+        rho = api.vertex_density(dm, dtype_w, prep.rho_bar)
+        return api.sample(dm, rho, dtype_vel, None, FIELDS, spec, method=2, samples=args.samples, seed=mc_seed)
+
+    ctx = api.context(dev)
+    log("warmup")
+    for _ in range(max(1, args.warmup)):
+        out = step(vel, w)
+    torch.cuda.synchronize()
+    deferred = ctx.last_deferred()
+    log(f"timed steps: {args.steps}")
+    kern = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(vel, w)
+        kern.append(ctx.last_timings())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    kern = np.array(kern)
+    k_sample_ms = float(kern[:, 0].mean())
+    k_exact_ms = float(kern[:, 1].mean())
+    k_seed_ms = float(kern[:, 2].mean())
+    k_vd_ms = float(kern[:, 3].mean())
+    n_cells = int(np.prod(n))
+    samples = n_cells * args.samples
+    value = samples / (ms_step * 1e-3)
+
+    # algorithmic bytes of the sampling kernel: tetrahedron ids, vertex
+    # positions / densities / velocities read once, outputs written once
+    esz = 4 if dt == torch.float32 else 8
+    B_alg = 16 * m.n_cells + m.n_vertices * (24 + esz + 3 * esz) + esz * OUT_FLOATS * n_cells
+    achieved = B_alg / (k_sample_ms * 1e-3) / 1e9
+
+    # all-FP64 comparison path on the same mesh
+    fp64 = None
+    if not args.no_fp64 and dt == torch.float32:
+        w64 = w.double()
+        v64 = vel.double()
+        for _ in range(1):
+            o64 = step(v64, w64)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        k64 = []
+        for _ in range(max(1, args.steps // 2)):
+            o64 = step(v64, w64)
+            k64.append(ctx.last_timings())
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        d32 = out["density"].double()
+        d64 = o64["density"]
+        nz = d64 > 0
+        rel = float(((d32[nz] - d64[nz]).abs() / d64[nz]).max().item())
+        fp64 = {"ms_per_step": (tb - ta) * 1e3 / max(1, args.steps // 2),
+                "sample_kernel_ms": float(np.mean([k[0] for k in k64])),
+                "mixed_vs_fp64_max_rel_density_err": rel}
+        del o64, w64, v64
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, prep, m, box, n, mc_seed)
+
+    res = {
+        "metric": "DTFE grid samples/s (256^3 grid, 8 MC samples per cell)",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": f"clustered_velocity N={args.n_side}^3 L=200 2000 blobs; {args.grid}^3 grid, "
+                        f"{args.samples} MC samples/cell; density+velocity+gradient+divergence+vorticity",
+            "n_particles": ps.n, "n_vertices": m.n_vertices, "n_tetrahedra": m.n_cells,
+            "grid": list(n), "samples_per_cell": args.samples, "geometry_dtype": "f64",
+            "field_dtype": args.dtype, "tessellation": m.backend,
+            "tessellation_s": round(m.seconds, 3),
+        },
+        "kernels_ms": {"sample": k_sample_ms, "exact": k_exact_ms, "seed_grid": k_seed_ms,
+                       "vertex_density": k_vd_ms},
+        "deferred_rows": deferred,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "k_sample_mc", "algorithmic_bytes": B_alg},
+        "fp64_path": fp64,
+        "fwd_adjoint_ratio": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(args, prep, m, box, n, mc_seed):
+    """The CPU oracle (FP64, one thread) on the first x-slabs of the grid."""
+    from oracle import oracle as O
+    from dtfe_amd import mesh as M
+    w = M.vertex_data(m, prep.weight)
+    vel = M.vertex_data(m, prep.vel)
+    t0 = time.perf_counter()
+    rho, _, _ = O.vertex_density(m, w, prep.rho_bar)
+    t_vd = time.perf_counter() - t0
+    dx = (box[1] - box[0]) / n[0]
+    slabs = max(1, args.cpu_slabs)
+    sub = (box[0], box[0] + slabs * dx, box[2], box[3], box[4], box[5])
+    t0 = time.perf_counter()
+    O.sample(m, rho, vel, None, fields=("density", "velocity", "velocity_gradient", "velocity_divergence",
+                                        "velocity_vorticity"), kind="grid", box=sub, n=(slabs, n[1], n[2]),
+             method=2, samples=args.samples, seed=mc_seed)
+    t_s = time.perf_counter() - t0
+    ns = slabs * n[1] * n[2] * args.samples
+    return {"value": ns / t_s, "unit": "samples/s", "cores": 1, "kind": "port",
+            "sample": f"{slabs} x-slab(s) of the {n[0]}^3 grid = {ns} MC samples, FP64 oracle "
+                      f"(per-sample walk + inverse, as the reference); vertex density of all "
+                      f"{m.n_vertices} vertices took {t_vd:.2f} s separately",
+            "seconds": t_s, "host_cpu": platform.processor() or platform.machine(),
+            "host_threads_available": os.cpu_count()}
+
+
+if __name__ == "__main__":
+    main()

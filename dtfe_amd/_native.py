@@ -48,7 +48,7 @@ METHOD_EQUI = 3
 EXPORTED = [
     "dtfe_hip_create", "dtfe_hip_destroy", "dtfe_hip_set_stream", "dtfe_hip_synchronize",
     "dtfe_hip_last_error", "dtfe_hip_vertex_density", "dtfe_hip_sample", "dtfe_hip_deposit",
-    "dtfe_hip_version",
+    "dtfe_hip_version", "dtfe_hip_last_timings", "dtfe_hip_last_deferred", "dtfe_hip_walk_failures",
 ]
 
 
@@ -116,6 +116,7 @@ def load(path: str = LIB_PATH):
                                          C.POINTER(C.c_int64 * 3), C.c_double, C.c_uint, C.POINTER(Outputs)]
         lib.dtfe_hip_last_deferred.argtypes = [vp]
         lib.dtfe_hip_walk_failures.argtypes = [vp, C.POINTER(C.c_int)]
+        lib.dtfe_hip_last_timings.argtypes = [vp, C.POINTER(C.c_double * 4)]
         for name in EXPORTED:
             if name not in ("dtfe_hip_last_error", "dtfe_hip_version"):
                 getattr(lib, name).restype = C.c_int
@@ -145,6 +146,12 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.dtfe_hip_synchronize(self.h))
+
+    def last_timings(self):
+        """(sample kernel, exact kernel, seed grid, vertex density) in ms."""
+        t = (C.c_double * 4)()
+        self.check(self.lib.dtfe_hip_last_timings(self.h, C.byref(t)))
+        return tuple(t)
 
     def last_deferred(self) -> int:
         return int(self.lib.dtfe_hip_last_deferred(self.h))

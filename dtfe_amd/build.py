@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libdtfe_hip.so")
+HOST_LIB = os.path.join(HERE, "libdtfe_host.so")
+HOST_SOURCES = ["delaunay.cpp"]
 
 HIP_SOURCES = ["dtfe_hip.hip"]
 HIP_HEADERS = ["predicates.h", "mt19937.h"]
@@ -44,7 +46,20 @@ def build_hip(verbose: bool = True) -> str:
     return LIB
 
 
+def build_host(verbose: bool = True) -> str:
+    """Host-side native code (the exact Delaunay tessellator) with g++."""
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
+    cmd = [cxx, "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", HOST_LIB + ".tmp", *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    return HOST_LIB
+
+
 def build_all(verbose: bool = True) -> None:
+    build_host(verbose)
     build_hip(verbose)
 
 

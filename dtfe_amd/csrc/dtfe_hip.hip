@@ -997,6 +997,10 @@ struct dtfe_ctx {
     DevBuf W, hull, seed_a, seed_b, bbox_part, bbox, status, axes, trig, equi, plan, flags, rows, mass, mom,
         defer_rows, defer_words;
     int last_deferred = 0;
+    // HIP events around the main kernels of the last calls (on ctx->stream)
+    hipEvent_t ev[8] = {};
+    bool ev_pending[4] = {false, false, false, false};
+    double last_ms[4] = {0, 0, 0, 0};  // 0 sample main kernel, 1 exact kernel, 2 seed grid, 3 vertex density
     // MC plan cache
     bool plan_valid = false;
     uint32_t plan_seed = 0;
@@ -1138,6 +1142,11 @@ int dtfe_hip_create(int device, dtfe_ctx **out) {
         return DTFE_ERR_HIP;
     }
     ctx->stream = ctx->own;
+    for (int i = 0; i < 8; ++i)
+        if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
+            delete ctx;
+            return DTFE_ERR_HIP;
+        }
     *out = ctx;
     return DTFE_OK;
 }
@@ -1151,6 +1160,8 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
+    for (int i = 0; i < 8; ++i)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return DTFE_OK;
@@ -1195,6 +1206,7 @@ int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *m, int dtype, const 
     hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->W.p, nv, 0.0);
     hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (uint8_t *)ctx->hull.p, nv,
                        (uint8_t)0);
+    HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
     hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
                        (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
                        (uint8_t *)ctx->hull.p);
@@ -1207,6 +1219,8 @@ int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *m, int dtype, const 
         hipLaunchKernelGGL(k_vertex_density<float>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
                            (const float *)weight, (const double *)ctx->W.p, (const uint8_t *)ctx->hull.p, nv, factor,
                            (float *)rho_out);
+    HIP_TRY(hipEventRecord(ctx->ev[7], ctx->stream));
+    ctx->ev_pending[3] = true;
     HIP_TRY(hipGetLastError());
     return DTFE_OK;
 }
@@ -1407,7 +1421,9 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
     }
 
     // ---- seed grid + status --------------------------------------------------------------
+    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
     if ((rc = build_seed_grid(ctx, m, a.sg))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     if ((rc = ensure(ctx, ctx->status, 16))) return rc;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->status.p, (int64_t)4, 0);
     a.status = (int *)ctx->status.p;
@@ -1461,17 +1477,28 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
     a.defer_words = (uint32_t *)ctx->defer_words.p;
     a.defer_cap = (int)cap;
     const bool f64 = vf->dtype == DTFE_F64;
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     launch_any(a, f64, want_vel, ns, 0, is_mc, grid, lds, 0, ctx->stream);
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
     HIP_TRY(hipGetLastError());
     int st[4];
     HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->last_ms[0] = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    ctx->last_ms[2] = ms;
+    ctx->last_ms[1] = 0.0;
     ctx->last_deferred = st[1];
     if (st[2]) {
         // queue overflow: redo everything with exact signs
         launch_any(a, f64, want_vel, ns, 2, is_mc, grid, lds, 0, ctx->stream);
     } else if (st[1] > 0) {
+        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
         launch_any(a, f64, want_vel, ns, 1, is_mc, grid_for(st[1]), 0, st[1], ctx->stream);
+        HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+        ctx->ev_pending[1] = true;
     }
     HIP_TRY(hipGetLastError());
     if (st[1] > 0 || st[2]) {
@@ -1560,6 +1587,27 @@ int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, i
 
 // Rows of the last sampling call that needed exact signs.
 int dtfe_hip_last_deferred(dtfe_ctx *ctx) { return ctx ? ctx->last_deferred : -1; }
+
+// Kernel times of the last calls, measured with HIP events on the context's
+// stream (synchronises it): ms[0] main sampling kernel, ms[1] exact-sign
+// kernel, ms[2] seed-grid build, ms[3] vertex density.
+int dtfe_hip_last_timings(dtfe_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return DTFE_ERR_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float t = 0.f;
+    if (ctx->ev_pending[1]) {
+        HIP_TRY(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+        ctx->last_ms[1] = t;
+        ctx->ev_pending[1] = false;
+    }
+    if (ctx->ev_pending[3]) {
+        HIP_TRY(hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
+        ctx->last_ms[3] = t;
+        ctx->ev_pending[3] = false;
+    }
+    for (int i = 0; i < 4; ++i) ms[i] = ctx->last_ms[i];
+    return DTFE_OK;
+}
 
 // Number of walks that did not terminate in the last sampling call
 // (synchronises the stream). Not part of the drop-in surface.

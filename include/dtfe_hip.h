@@ -159,6 +159,16 @@ int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, i
 /* Library version string. */
 const char *dtfe_hip_version(void);
 
+/* ---- diagnostics (no counterpart in the reference) ------------------------------ */
+/* Kernel times of the last calls, from HIP events on the context's stream
+ * (synchronises it): ms[0] main sampling kernel, ms[1] exact-sign kernel,
+ * ms[2] seed-grid build, ms[3] vertex density. */
+int dtfe_hip_last_timings(dtfe_ctx *ctx, double ms[4]);
+/* Rows of the last dtfe_hip_sample call that needed exact signs. */
+int dtfe_hip_last_deferred(dtfe_ctx *ctx);
+/* Point locations of the last dtfe_hip_sample call that did not terminate. */
+int dtfe_hip_walk_failures(dtfe_ctx *ctx, int *failures);
+
 #ifdef __cplusplus
 }
 #endif

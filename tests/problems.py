@@ -21,7 +21,7 @@ class Problem:
     region: tuple
 
 
-def _make(ps, backend="qhull"):
+def _make(ps, backend="native"):
     prep = M.prepare_particles(ps)
     m = M.tessellate(prep.pos, backend=backend)
     return Problem(ps=ps, prep=prep, mesh=m, w=M.vertex_data(m, prep.weight), vel=M.vertex_data(m, prep.vel),
@@ -37,6 +37,19 @@ def uniform_problem(n=3000, seed=3):
 def clustered_problem(n_side=16, seed=4):
     ps = S.clustered(n_side ** 3, 20.0, 20, 0.2, 1.0, seed, with_velocity=True, lognormal_weights=True,
                      n_scalars=2)
+    return _make(ps)
+
+
+@functools.lru_cache(maxsize=None)
+def lattice_problem(k=6):
+    """Particles on a regular lattice: cospherical points, and cell-centre
+    samples on faces, edges and vertices of the tessellation."""
+    g = np.arange(k) + 0.5
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    rng = np.random.default_rng(9)
+    ps = S.ParticleSet(pos=pos, weight=np.ones(pos.shape[0]), vel=rng.standard_normal(pos.shape),
+                       box=(0.0, float(k), 0.0, float(k), 0.0, float(k)), periodic=True)
     return _make(ps)
 
 
