@@ -106,6 +106,7 @@ def main():
         out = step(vel, w)
     torch.cuda.synchronize()
     deferred = ctx.last_deferred()
+    walk = ctx.last_walk_stats()
     log(f"timed steps: {args.steps}")
     kern = []
     torch.cuda.synchronize()
@@ -183,6 +184,8 @@ def main():
         "kernels_ms": {"sample": k_sample_ms, "exact": k_exact_ms, "seed_grid": k_seed_ms,
                        "vertex_density": k_vd_ms},
         "deferred_rows": deferred,
+        "walk_steps": {"centre_per_row": walk[0] / max(1, walk[2]), "per_sample": walk[1] / max(1, walk[3])}
+        if walk[0] else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "k_sample_mc", "algorithmic_bytes": B_alg},

@@ -49,6 +49,7 @@ EXPORTED = [
     "dtfe_hip_create", "dtfe_hip_destroy", "dtfe_hip_set_stream", "dtfe_hip_synchronize",
     "dtfe_hip_last_error", "dtfe_hip_vertex_density", "dtfe_hip_sample", "dtfe_hip_deposit",
     "dtfe_hip_version", "dtfe_hip_last_timings", "dtfe_hip_last_deferred", "dtfe_hip_walk_failures",
+    "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint",
 ]
 
 
@@ -79,6 +80,17 @@ class Outputs(C.Structure):
                 ("velocity_vorticity", C.c_void_p), ("velocity_std", C.c_void_p),
                 ("scalar", C.c_void_p), ("scalar_gradient", C.c_void_p),
                 ("located_cell", C.c_void_p), ("n_written", C.c_int64)]
+
+
+class Upstream(C.Structure):
+    _fields_ = [("density", C.c_void_p), ("velocity", C.c_void_p), ("velocity_gradient", C.c_void_p),
+                ("velocity_divergence", C.c_void_p), ("velocity_shear", C.c_void_p),
+                ("velocity_vorticity", C.c_void_p)]
+
+
+class VertexGrads(C.Structure):
+    _fields_ = [("position", C.c_void_p), ("weight", C.c_void_p), ("velocity", C.c_void_p),
+                ("rho_bar", C.c_double)]
 
 
 _lib = None
@@ -117,6 +129,10 @@ def load(path: str = LIB_PATH):
         lib.dtfe_hip_last_deferred.argtypes = [vp]
         lib.dtfe_hip_walk_failures.argtypes = [vp, C.POINTER(C.c_int)]
         lib.dtfe_hip_last_timings.argtypes = [vp, C.POINTER(C.c_double * 4)]
+        lib.dtfe_hip_last_walk_stats.argtypes = [vp, C.POINTER(C.c_int64 * 4)]
+        lib.dtfe_hip_adjoint.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), vp, C.c_double,
+                                         C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Upstream),
+                                         C.POINTER(VertexGrads)]
         for name in EXPORTED:
             if name not in ("dtfe_hip_last_error", "dtfe_hip_version"):
                 getattr(lib, name).restype = C.c_int
@@ -152,6 +168,12 @@ class Context:
         t = (C.c_double * 4)()
         self.check(self.lib.dtfe_hip_last_timings(self.h, C.byref(t)))
         return tuple(t)
+
+    def last_walk_stats(self):
+        """(centre-walk steps, sample-walk steps, rows, samples) of the last call."""
+        t = (C.c_int64 * 4)()
+        self.check(self.lib.dtfe_hip_last_walk_stats(self.h, C.byref(t)))
+        return tuple(int(x) for x in t)
 
     def last_deferred(self) -> int:
         return int(self.lib.dtfe_hip_last_deferred(self.h))

@@ -216,3 +216,76 @@ def deposit(scheme: str, pos: torch.Tensor, weight: torch.Tensor, vel: Optional[
                                   float(rho_bar), bits, C.byref(o))
     ctx.check(rc)
     return out
+
+
+UPSTREAM_FIELDS = ("density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_shear",
+                   "velocity_vorticity")
+
+
+def _spec_struct(spec, keep):
+    sp = N.Sampling()
+    if isinstance(spec, GridSpec):
+        sp.kind = N.SAMPLE_GRID
+        sp.box[:] = [float(x) for x in spec.box]
+        sp.n[:] = [int(x) for x in spec.n]
+    elif isinstance(spec, ConeSpec):
+        sp.kind = N.SAMPLE_CONE
+        sp.cone[:] = [float(x) for x in spec.cone]
+        sp.n[:] = [int(x) for x in spec.n]
+        sp.origin[:] = [float(x) for x in spec.origin]
+    elif isinstance(spec, PointsSpec):
+        sp.kind = N.SAMPLE_POINTS
+        sp.box[:] = [float(x) for x in spec.box]
+        pts = spec.points.contiguous()
+        keep.append(pts)
+        sp.n_points = pts.shape[0]
+        sp.points = pts.data_ptr()
+        if spec.deltas is not None:
+            dl = spec.deltas.contiguous()
+            keep.append(dl)
+            sp.deltas = dl.data_ptr()
+    else:
+        raise TypeError("spec must be GridSpec, ConeSpec or PointsSpec")
+    return sp
+
+
+def adjoint(mesh: DeviceMesh, rho: torch.Tensor, weight: torch.Tensor, vel: Optional[torch.Tensor],
+            rho_bar: float, spec, upstream: Dict[str, torch.Tensor], method: int = 0, samples: int = 1,
+            seed: int = 0, part_no: int = -1, arg_order: int = 0, value_bug: bool = True,
+            want=("position", "weight", "velocity")):
+    """Reverse mode of vertex_density + sample at fixed tessellation.
+
+    upstream maps output field names to dL/d(field) (same shapes as the
+    outputs of `sample`). Returns (grads dict with 'position' [nv, 3],
+    'weight' [nv], 'velocity' [nv, 3], and dL/d rho_bar as a float)."""
+    ctx = context(mesh.device)
+    dtype = rho.dtype
+    nv = mesh.pos.shape[0]
+    keep = []
+    sp = _spec_struct(spec, keep)
+    vf = N.VertexFields(_dtype_code(dtype), _ptr(rho.contiguous()), _ptr(None if vel is None else vel.contiguous()),
+                        None, 0)
+    up = N.Upstream()
+    for f in UPSTREAM_FIELDS:
+        if f in upstream and upstream[f] is not None:
+            g = upstream[f].to(dtype).contiguous()
+            keep.append(g)
+            setattr(up, f, g.data_ptr())
+    out = {}
+    gr = N.VertexGrads()
+    if "position" in want:
+        out["position"] = torch.empty((nv, 3), dtype=dtype, device=mesh.device)
+        gr.position = out["position"].data_ptr()
+    if "weight" in want:
+        out["weight"] = torch.empty(nv, dtype=dtype, device=mesh.device)
+        gr.weight = out["weight"].data_ptr()
+    if "velocity" in want and vel is not None:
+        out["velocity"] = torch.empty((nv, 3), dtype=dtype, device=mesh.device)
+        gr.velocity = out["velocity"].data_ptr()
+    mc = N.MC(int(method), int(samples), int(seed) & ((1 << 64) - 1), int(part_no), int(arg_order))
+    w = weight.to(dtype).contiguous()
+    cm = mesh.c()
+    rc = ctx.lib.dtfe_hip_adjoint(ctx.h, C.byref(cm), C.byref(vf), _ptr(w), float(rho_bar), C.byref(sp),
+                                  C.byref(mc), N.COMPAT_VALUE_BUG if value_bug else 0, C.byref(up), C.byref(gr))
+    ctx.check(rc, allow=(N.DTFE_ERR_TOO_FEW_VERTICES,))
+    return out, float(gr.rho_bar)

@@ -36,14 +36,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build dtfe_amd")
 
 
-def build_hip(verbose: bool = True) -> str:
+def build_hip(verbose: bool = True, defines=(), out: str = LIB) -> str:
+    """hipcc build of libdtfe_hip.so; `defines` are extra -D options."""
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", LIB + ".tmp", *srcs]
+    cmd = [hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"),
+           "-o", out + ".tmp", *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 def build_host(verbose: bool = True) -> str:
@@ -58,9 +60,40 @@ def build_host(verbose: bool = True) -> str:
     return HOST_LIB
 
 
+DROPIN = os.path.join(CSRC, "dropin")
+DROPIN_LIBS = {"float": os.path.join(HERE, "libDTFE.so"), "double": os.path.join(HERE, "libDTFE_double.so")}
+DROPIN_RUN = {"float": os.path.join(HERE, "dtfe_dropin_run"), "double": os.path.join(HERE, "dtfe_dropin_run_double")}
+
+
+def build_dropin(verbose: bool = True) -> None:
+    """The C++ DTFE() drop-in (libDTFE.so with Real = float, libDTFE_double.so
+    with -DDOUBLE) and its command-line driver, linked to the two C ABIs."""
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    rocm = "/opt/rocm"
+    common = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include",
+              f"-I{DROPIN}"]
+    for real, lib in DROPIN_LIBS.items():
+        defs = ["-DDOUBLE"] if real == "double" else []
+        cmd = [cxx, *common, *defs, "-shared", "-o", lib + ".tmp", os.path.join(DROPIN, "DTFE.cpp"),
+               f"-L{HERE}", "-l:libdtfe_hip.so", "-l:libdtfe_host.so", f"-L{rocm}/lib", "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{rocm}/lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(lib + ".tmp", lib)
+        exe = DROPIN_RUN[real]
+        cmd = [cxx, *common, *defs, "-o", exe + ".tmp", os.path.join(DROPIN, "dtfe_dropin_run.cpp"),
+               f"-L{HERE}", f"-l:{os.path.basename(lib)}", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(exe + ".tmp", exe)
+
+
 def build_all(verbose: bool = True) -> None:
     build_host(verbose)
     build_hip(verbose)
+    build_dropin(verbose)
 
 
 if __name__ == "__main__":

@@ -601,6 +601,67 @@ int dtfe_tess_copy(const dtfe_tess *t, int32_t *cells, int32_t *nbrs, int32_t *d
 
 void dtfe_tess_free(dtfe_tess *t) { delete t; }
 
+// Renumbers the vertices of a mesh along a Morton curve and sorts its
+// tetrahedra by the Morton code of their centroids, in place; neighbour
+// indices follow. vertex_perm[new] = old. Spatially close vertices and
+// tetrahedra then share cache lines, which is what the gathers, atomics and
+// walks of the GPU kernels need.
+int dtfe_mesh_spatial_order(int64_t nv, double *pos, int64_t nc, int32_t *cells, int32_t *nbrs,
+                            int64_t *vertex_perm) {
+    if (nv < 0 || nc < 0 || (nv && !pos) || (nc && (!cells || !nbrs))) return 1;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = 0; i < nv; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], pos[3 * i + k]);
+            hi[k] = std::max(hi[k], pos[3 * i + k]);
+        }
+    double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    auto code_of = [&](const double *p) {
+        uint64_t c[3];
+        for (int k = 0; k < 3; ++k) {
+            double f = (p[k] - lo[k]) / ext;
+            c[k] = (uint64_t)std::min(2097151.0, std::max(0.0, f * 2097152.0));
+        }
+        return spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+    };
+    // vertices
+    std::vector<std::pair<uint64_t, int64_t>> vk(nv);
+    for (int64_t i = 0; i < nv; ++i) vk[i] = {code_of(pos + 3 * i), i};
+    std::sort(vk.begin(), vk.end());
+    std::vector<int32_t> new_of(nv);
+    std::vector<double> npos(3 * nv);
+    for (int64_t j = 0; j < nv; ++j) {
+        int64_t old = vk[j].second;
+        new_of[old] = (int32_t)j;
+        if (vertex_perm) vertex_perm[j] = old;
+        for (int k = 0; k < 3; ++k) npos[3 * j + k] = pos[3 * old + k];
+    }
+    std::memcpy(pos, npos.data(), npos.size() * sizeof(double));
+    for (int64_t i = 0; i < 4 * nc; ++i) cells[i] = new_of[cells[i]];
+    // tetrahedra by centroid
+    std::vector<std::pair<uint64_t, int64_t>> tk(nc);
+    for (int64_t t = 0; t < nc; ++t) {
+        double c[3] = {0, 0, 0};
+        for (int i = 0; i < 4; ++i)
+            for (int k = 0; k < 3; ++k) c[k] += 0.25 * pos[3 * (int64_t)cells[4 * t + i] + k];
+        tk[t] = {code_of(c), t};
+    }
+    std::sort(tk.begin(), tk.end());
+    std::vector<int32_t> tnew(nc), nc_cells(4 * nc), nc_nbrs(4 * nc);
+    for (int64_t j = 0; j < nc; ++j) tnew[tk[j].second] = (int32_t)j;
+    for (int64_t j = 0; j < nc; ++j) {
+        int64_t old = tk[j].second;
+        for (int i = 0; i < 4; ++i) {
+            nc_cells[4 * j + i] = cells[4 * old + i];
+            int32_t n = nbrs[4 * old + i];
+            nc_nbrs[4 * j + i] = n < 0 ? -1 : tnew[n];
+        }
+    }
+    std::memcpy(cells, nc_cells.data(), nc_cells.size() * sizeof(int32_t));
+    std::memcpy(nbrs, nc_nbrs.data(), nc_nbrs.size() * sizeof(int32_t));
+    return 0;
+}
+
 // Exact predicate signs, exported for tests.
 int dtfe_orient_sign(const double *a, const double *b, const double *c, const double *d) {
     return orient_sign(a, b, c, d);

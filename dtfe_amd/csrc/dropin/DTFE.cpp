@@ -1,0 +1,504 @@
+// DTFE() drop-in: the control flow of the reference's library entry
+// (src/DTFE.cpp:76-251, interpolate :263-281, DTFE_interpolation
+// src/CGAL_triangulation/triangulation.cpp:66-145) with the tessellation from
+// the native exact tessellator and every step after it on the GPU through
+// the C ABI of libdtfe_hip.so.
+//
+// One partition: the GPU replaces the reference's OpenMP split
+// (src/DTFE.cpp:287-385), so results equal the reference's single-thread run.
+// Not provided: --poisson / --randomSample input generation (src/random.cc),
+// --partition serial chunking, averaging method 1 (GSL Sobol), NGP and SPH;
+// asking for them is a fatal error, as an invalid option is in the reference.
+#include "DTFE.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+#include "../../../include/dtfe_hip.h"
+
+extern "C" {
+struct dtfe_tess;
+int dtfe_tess_build(int64_t n, const double *pts, dtfe_tess **out);
+int64_t dtfe_tess_n_cells(const dtfe_tess *t);
+int dtfe_tess_copy(const dtfe_tess *t, int32_t *cells, int32_t *nbrs, int32_t *dup_of);
+void dtfe_tess_free(dtfe_tess *t);
+int dtfe_mesh_spatial_order(int64_t nv, double *pos, int64_t nc, int32_t *cells, int32_t *nbrs,
+                            int64_t *vertex_perm);
+}
+
+namespace dtfe_dropin {
+namespace {
+bool g_throw = false;
+double g_tess_seconds = 0.0;
+}  // namespace
+
+void throw_on_error(bool enable) { g_throw = enable; }
+double last_tessellation_seconds() { return g_tess_seconds; }
+
+[[noreturn]] void fatal(const std::string &msg) {
+    if (g_throw) throw Error(msg);
+    std::fprintf(stderr, "\n~~~ ERROR ~~~ %s\n", msg.c_str());
+    std::exit(EXIT_FAILURE);
+}
+
+void warn(const User_options &o, const std::string &msg) {
+    if (o.verboseLevel >= 1) std::fprintf(stdout, "~~~ WARNING ~~~ %s\n", msg.c_str());
+}
+}  // namespace dtfe_dropin
+
+using dtfe_dropin::fatal;
+
+// ---------------------------------------------------------------------------
+// option updates (src/user_options.cc:926-1089)
+// ---------------------------------------------------------------------------
+void User_options::updateFullBox(Box &newBox) {
+    boxCoordinates = newBox;
+    fullBoxOffset.assign(NO_DIM, Real(0.));
+    fullBoxLength.assign(NO_DIM, Real(0.));
+    for (size_t i = 0; i < NO_DIM; ++i) {
+        fullBoxOffset[i] = boxCoordinates[2 * i];
+        fullBoxLength[i] = boxCoordinates[2 * i + 1] - boxCoordinates[2 * i];
+    }
+}
+
+void User_options::updatePadding(size_t noParticles) {
+    if (paddingLength.isNullBox()) {
+        if (paddingParticles <= Real(0.)) {
+            paddingLength.assign(Real(0.));
+        } else if (DTFE || SPH) {
+            for (int i = 0; i < 2 * NO_DIM; ++i)
+                paddingLength[i] = paddingParticles / Real(std::pow((double)noParticles, 1. / NO_DIM)) *
+                                   fullBoxLength[i % 2];
+        } else if (TSC || NGP) {
+            for (int i = 0; i < 2 * NO_DIM; ++i) paddingLength[i] = fullBoxLength[i % 2] / gridSize[i % 2];
+        } else if (CIC) {
+            paddingLength.assign(Real(0.));
+        }
+        paddingOn = true;
+        paddingMpcOn = true;
+    } else if (paddingOn && !paddingMpcOn) {
+        for (int i = 0; i < 2 * NO_DIM; ++i) paddingLength[i] *= fullBoxLength[i % 2];
+        paddingMpcOn = true;
+    }
+}
+
+void User_options::updateEntries(size_t noTotalParticles, bool userSampling) {
+    if (boxCoordinates.size() != 2 * NO_DIM) fatal("the data box must have 6 coordinates");
+    if ((gridSize.empty() && !userSampling) || (!gridSize.empty() && gridSize.size() != NO_DIM))
+        fatal("the interpolation grid must have 3 values");
+    for (size_t g : gridSize)
+        if (g < 1) fatal("the values of option '--grid' must be >= 1");
+    paddedBox = boxCoordinates;
+    updateFullBox(boxCoordinates);
+    updatePadding(noTotalParticles);
+    if (regionOn && !regionMpcOn) {
+        for (size_t i = 0; i < region.size(); ++i)
+            region[i] = fullBoxOffset[i % 2] + region[i] * fullBoxLength[i % 2];  // i%2 as the reference
+        regionMpcOn = true;
+    } else if (!regionOn) {
+        region = boxCoordinates;
+    }
+    userDefinedSampling = userSampling;
+    if (userSampling) {
+        redshiftConeOn = false;
+        if (method != 2 && aField.selected()) {
+            method = 2;
+            if (!noPointsOn) noPoints = 20;
+            dtfe_dropin::warn(*this, "user sampling points: volume averaging uses method 2 (Monte Carlo)");
+        }
+    } else if (redshiftConeOn) {
+        if (method != 2 && aField.selected()) {
+            method = 2;
+            if (!noPointsOn) noPoints = 20;
+            dtfe_dropin::warn(*this, "redshift cone grid: volume averaging uses method 2 (Monte Carlo)");
+        }
+    }
+    if (partitionOn && (userSampling || redshiftConeOn)) {
+        partitionOn = false;
+        for (size_t i = 0; i < partition.size(); ++i) partition[i] = 1;
+        partNo = -1;
+    }
+    if (redshiftConeOn) {
+        if (redshiftCone[2] < 0 || redshiftCone[2] > 180 || redshiftCone[3] < 0 || redshiftCone[3] > 180)
+            fatal("the theta limits of option '--redshiftCone' must be in [0, 180]");
+        if (redshiftCone[5] - redshiftCone[4] >= Real(360.)) fatal("the 'psi' angle interval can stretch at most 360 degrees");
+        if (originPosition.size() != NO_DIM) fatal("'originPosition' must have 3 entries");
+    }
+    if (randomSample >= Real(0.) && randomSample > Real(1.)) fatal("'--randomSample' must be in [0, 1]");
+    if (method == 3) {
+        int n = (int)std::lround(std::cbrt((double)noPoints));
+        if (n * n * n != noPoints) fatal("averaging method 3 needs a perfect cube of sample points");
+    }
+    if (NGP || CIC || TSC || SPH) {
+        if (aField.velocity_gradient || aField.selectedVelocityDerivatives())
+            fatal("the NGP, CIC, TSC and SPH methods do not compute the velocity gradient");
+        if (aField.scalar_gradient) fatal("the NGP, CIC, TSC and SPH methods do not compute scalar gradients");
+        if ((CIC || TSC) && aField.scalar) fatal("the NGP, CIC and TSC methods do not interpolate scalars");
+    }
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// host steps before the tessellation
+// ---------------------------------------------------------------------------
+// averageDensity (src/subpartition.h:437-449): double sum / Box::volume()
+Real averageDensity(std::vector<Particle_data> &p, const User_options &o) {
+    double total = 0.;
+    for (auto &q : p) total += q.weight();
+    return Real(total / o.boxCoordinates.volume());
+}
+
+// findParticlesInBox / insertParticlesInBox (src/subpartition.h:79-154)
+void findParticlesInBox(std::vector<Particle_data> &p, std::vector<Particle_data> *out, const User_options &o) {
+    const Box &box = o.paddedBox;
+    const Box &full = o.boxCoordinates;
+    Real L[NO_DIM];
+    for (size_t i = 0; i < NO_DIM; ++i) L[i] = full[2 * i + 1] - full[2 * i];
+    out->clear();
+    if (!o.periodic) {
+        for (auto &q : p)
+            if (box.isParticleInBox(q)) out->push_back(q);
+        return;
+    }
+    for (int i1 = 0; i1 < 3; ++i1)
+        for (int i2 = 0; i2 < 3; ++i2)
+            for (int i3 = 0; i3 < 3; ++i3) {
+                Real off[3] = {Real(i1 - 1) * L[0], Real(i2 - 1) * L[1], Real(i3 - 1) * L[2]};
+                Box nb = box;
+                nb.translate(off);
+                for (auto &q : p)
+                    if (nb.isParticleInBox(q)) {
+                        Particle_data t = q;
+                        for (int j = 0; j < NO_DIM; ++j) t.pos[j] -= off[j];
+                        out->push_back(t);
+                    }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// device memory
+// ---------------------------------------------------------------------------
+struct DevMem {
+    void *p = nullptr;
+    DevMem() = default;
+    explicit DevMem(size_t bytes) {
+        if (bytes && hipMalloc(&p, bytes) != hipSuccess) fatal("device allocation failed");
+    }
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    DevMem(DevMem &&o) noexcept : p(o.p) { o.p = nullptr; }
+    DevMem &operator=(DevMem &&o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p;
+            o.p = nullptr;
+        }
+        return *this;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+template <typename T>
+DevMem upload(const T *h, size_t n) {
+    DevMem d(n * sizeof(T));
+    if (n && hipMemcpy(d.p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) fatal("host to device copy failed");
+    return d;
+}
+
+template <typename T>
+void download(std::vector<T> &dst, const DevMem &src, size_t n) {
+    dst.resize(n);
+    if (n && hipMemcpy(dst.data(), src.p, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+        fatal("device to host copy failed");
+}
+
+struct Ctx {
+    dtfe_ctx *h = nullptr;
+    explicit Ctx(int device) {
+        if (dtfe_hip_create(device, &h) != DTFE_OK) fatal("cannot create a GPU context on device " + std::to_string(device));
+    }
+    ~Ctx() { dtfe_hip_destroy(h); }
+    void check(int rc, const char *what) {
+        if (rc == DTFE_OK) return;
+        if (rc == DTFE_ERR_TOO_FEW_VERTICES) return;  // fields are zero-filled, as the reference does
+        fatal(std::string(what) + ": " + dtfe_hip_last_error(h));
+    }
+};
+
+constexpr int kDtype = sizeof(Real) == 8 ? DTFE_F64 : DTFE_F32;
+
+unsigned field_bits(const Field &f) {
+    unsigned b = 0;
+    if (f.density) b |= DTFE_FIELD_DENSITY;
+    if (f.velocity) b |= DTFE_FIELD_VELOCITY;
+    if (f.velocity_gradient) b |= DTFE_FIELD_VELOCITY_GRADIENT;
+    if (f.velocity_divergence) b |= DTFE_FIELD_VELOCITY_DIVERGENCE;
+    if (f.velocity_shear) b |= DTFE_FIELD_VELOCITY_SHEAR;
+    if (f.velocity_vorticity) b |= DTFE_FIELD_VELOCITY_VORTICITY;
+    if (f.velocity_std) b |= DTFE_FIELD_VELOCITY_STD;
+    if (f.scalar) b |= DTFE_FIELD_SCALAR;
+    if (f.scalar_gradient) b |= DTFE_FIELD_SCALAR_GRADIENT;
+    return b;
+}
+
+// Runs one sampling pass and fills q with the requested fields.
+void sample_into(Ctx &ctx, const dtfe_mesh &mesh, const dtfe_vertex_fields &vf, const Field &f,
+                 const dtfe_sampling &smp, const dtfe_mc &mc, unsigned flags, size_t n_rows, Quantities *q) {
+    unsigned bits = field_bits(f);
+    if (!bits) return;
+    const size_t S = noScalarComp;
+    DevMem d_den(f.density ? n_rows * sizeof(Real) : 0), d_vel(f.velocity ? 3 * n_rows * sizeof(Real) : 0),
+        d_grad(f.velocity_gradient ? 9 * n_rows * sizeof(Real) : 0),
+        d_div(f.velocity_divergence ? n_rows * sizeof(Real) : 0),
+        d_shear(f.velocity_shear ? 5 * n_rows * sizeof(Real) : 0),
+        d_vort(f.velocity_vorticity ? 3 * n_rows * sizeof(Real) : 0),
+        d_std(f.velocity_std ? n_rows * sizeof(Real) : 0), d_sc(f.scalar ? S * n_rows * sizeof(Real) : 0),
+        d_scg(f.scalar_gradient ? 3 * S * n_rows * sizeof(Real) : 0);
+    dtfe_outputs out{};
+    out.density = d_den.p;
+    out.velocity = d_vel.p;
+    out.velocity_gradient = d_grad.p;
+    out.velocity_divergence = d_div.p;
+    out.velocity_shear = d_shear.p;
+    out.velocity_vorticity = d_vort.p;
+    out.velocity_std = d_std.p;
+    out.scalar = d_sc.p;
+    out.scalar_gradient = d_scg.p;
+    ctx.check(dtfe_hip_sample(ctx.h, &mesh, &vf, bits, &smp, &mc, flags, &out), "DTFE sampling");
+    ctx.check(dtfe_hip_synchronize(ctx.h), "DTFE sampling");
+    size_t n = (size_t)out.n_written;
+    if (f.density) download(q->density, d_den, n);
+    if (f.velocity) download(q->velocity, d_vel, n);
+    if (f.velocity_gradient) download(q->velocity_gradient, d_grad, n);
+    if (f.velocity_divergence) download(q->velocity_divergence, d_div, n);
+    if (f.velocity_shear) download(q->velocity_shear, d_shear, n);
+    if (f.velocity_vorticity) download(q->velocity_vorticity, d_vort, n);
+    if (f.velocity_std) download(q->velocity_std, d_std, n);
+    if (f.scalar) download(q->scalar, d_sc, n);
+    if (f.scalar_gradient) download(q->scalar_gradient, d_scg, n);
+}
+
+void dtfe_interpolation(std::vector<Particle_data> *p, std::vector<Sample_point> &samples, User_options &o,
+                        Quantities *uQ, Quantities *aQ) {
+    if (o.method < 1 || o.method > 3) fatal("'method' must have values from 1 to 3");
+    // ---- tessellation (triangulation.cpp:162-195); duplicates keep the last copy's data
+    const size_t np = p->size();
+    std::vector<size_t> order(np);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        for (int k = 0; k < 3; ++k) {
+            double x = (*p)[a].pos[k], y = (*p)[b].pos[k];
+            if (x != y) return x < y;
+        }
+        return false;
+    });
+    std::vector<uint8_t> keep(np, 1);
+    for (size_t i = 0; i + 1 < np; ++i) {
+        auto &a = (*p)[order[i]], &b = (*p)[order[i + 1]];
+        if (a.pos[0] == b.pos[0] && a.pos[1] == b.pos[1] && a.pos[2] == b.pos[2]) keep[order[i]] = 0;
+    }
+    std::vector<double> pos;
+    std::vector<Real> w, vel, sc;
+    pos.reserve(3 * np);
+    for (size_t i = 0; i < np; ++i) {
+        if (!keep[i]) continue;
+        auto &q = (*p)[i];
+        for (int k = 0; k < 3; ++k) pos.push_back((double)q.pos[k]);
+        w.push_back(q.weight());
+        for (int k = 0; k < 3; ++k) vel.push_back(q.velocity()[k]);
+        for (size_t k = 0; k < noScalarComp; ++k) sc.push_back(q.scalar()[k]);
+    }
+    p->clear();
+    const int64_t nv = (int64_t)w.size();
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int32_t> cells, nbrs;
+    if (nv >= 4) {
+        dtfe_tess *tess = nullptr;
+        int rc = dtfe_tess_build(nv, pos.data(), &tess);
+        if (rc == 1) fatal("tessellation: bad input");
+        int64_t nc = dtfe_tess_n_cells(tess);
+        cells.resize(4 * nc);
+        nbrs.resize(4 * nc);
+        if (nc) dtfe_tess_copy(tess, cells.data(), nbrs.data(), nullptr);
+        dtfe_tess_free(tess);
+        // Morton order of vertices and tetrahedra for the GPU gathers; vertex data follow
+        std::vector<int64_t> perm(nv);
+        dtfe_mesh_spatial_order(nv, pos.data(), nc, cells.data(), nbrs.data(), perm.data());
+        std::vector<Real> w2(nv), v2(3 * nv), s2(noScalarComp * nv);
+        for (int64_t j = 0; j < nv; ++j) {
+            int64_t o = perm[j];
+            w2[j] = w[o];
+            for (int k = 0; k < 3; ++k) v2[3 * j + k] = vel[3 * o + k];
+            for (size_t k = 0; k < noScalarComp; ++k) s2[noScalarComp * j + k] = sc[noScalarComp * o + k];
+        }
+        w.swap(w2);
+        vel.swap(v2);
+        sc.swap(s2);
+    }
+    dtfe_dropin::g_tess_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+
+    // ---- device mesh and vertex fields
+    Ctx ctx(o.device);
+    DevMem d_pos = upload(pos.data(), pos.size()), d_cells = upload(cells.data(), cells.size()),
+           d_nbrs = upload(nbrs.data(), nbrs.size()), d_w = upload(w.data(), w.size()),
+           d_vel = upload(vel.data(), vel.size()), d_sc = upload(sc.data(), sc.size()),
+           d_rho(std::max<size_t>(1, w.size()) * sizeof(Real));
+    dtfe_mesh mesh{nv, (int64_t)cells.size() / 4, d_pos.as<double>(), d_cells.as<int32_t>(), d_nbrs.as<int32_t>()};
+    if (nv < 4)
+        dtfe_dropin::warn(o, "fewer than 4 vertices in the Delaunay triangulation: all values are set to 0");
+    // ---- vertex density (triangulation.cpp:200-279)
+    ctx.check(dtfe_hip_vertex_density(ctx.h, &mesh, kDtype, d_w.p, (double)o.averageDensity, d_rho.p),
+              "vertex density");
+    dtfe_vertex_fields vf{kDtype, d_rho.p, d_vel.p, d_sc.p, (int)noScalarComp};
+    unsigned flags = o.referenceValueFormula ? DTFE_COMPAT_VALUE_BUG : 0u;
+
+    // sampling geometry
+    dtfe_sampling smp{};
+    size_t n_rows = 0;
+    std::vector<double> spts, sdel;
+    DevMem d_spts, d_sdel;
+    for (int i = 0; i < 6; ++i) smp.box[i] = o.region[i];
+    if (o.userDefinedSampling) {
+        smp.kind = DTFE_SAMPLE_POINTS;
+        for (auto &s : samples)
+            for (int k = 0; k < 3; ++k) {
+                spts.push_back(s.pos[k]);
+                sdel.push_back(s._delta[k]);
+            }
+        d_spts = upload(spts.data(), spts.size());
+        d_sdel = upload(sdel.data(), sdel.size());
+        smp.n_points = (int64_t)samples.size();
+        smp.points = d_spts.as<double>();
+        smp.deltas = d_sdel.as<double>();
+        n_rows = samples.size();
+    } else {
+        smp.kind = o.redshiftConeOn ? DTFE_SAMPLE_CONE : DTFE_SAMPLE_GRID;
+        for (int i = 0; i < 3; ++i) smp.n[i] = (int64_t)o.gridSize[i];
+        if (o.redshiftConeOn) {
+            for (int i = 0; i < 6; ++i) smp.cone[i] = o.redshiftCone[i];
+            for (int i = 0; i < 3; ++i) smp.origin[i] = o.originPosition[i];
+        }
+        n_rows = o.gridSize[0] * o.gridSize[1] * o.gridSize[2];
+    }
+    if (o.uField.selected()) {
+        dtfe_mc mc{DTFE_METHOD_CENTRE, 1, (uint64_t)o.randomSeed, o.partNo, 0};
+        sample_into(ctx, mesh, vf, o.uField, smp, mc, flags, n_rows, uQ);
+    }
+    if (o.aField.selected()) {
+        if (o.method == 1)
+            fatal("volume averaging method 1 (quasi-random Sobol samples) is not provided by the MI355X drop-in; "
+                  "use method 2 or 3");
+        if (o.method == 3 && smp.kind != DTFE_SAMPLE_GRID)
+            fatal("averaging method 3 is only available on a regular grid");
+        dtfe_mc mc{o.method == 2 ? DTFE_METHOD_MC : DTFE_METHOD_EQUI, o.noPoints, (uint64_t)o.randomSeed, o.partNo, 0};
+        sample_into(ctx, mesh, vf, o.aField, smp, mc, flags, n_rows, aQ);
+    }
+}
+
+void deposit(std::vector<Particle_data> *p, User_options &o, Quantities *aQ, int scheme) {
+    const size_t np = p->size();
+    std::vector<double> pos(3 * np);
+    std::vector<Real> w(np), vel(3 * np);
+    for (size_t i = 0; i < np; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * i + k] = (*p)[i].pos[k];
+            vel[3 * i + k] = (*p)[i].velocity()[k];
+        }
+        w[i] = (*p)[i].weight();
+    }
+    p->clear();
+    Ctx ctx(o.device);
+    DevMem d_pos = upload(pos.data(), pos.size()), d_w = upload(w.data(), w.size()),
+           d_vel = upload(vel.data(), vel.size());
+    size_t ncell = o.gridSize[0] * o.gridSize[1] * o.gridSize[2];
+    DevMem d_den(ncell * sizeof(Real)), d_v(3 * ncell * sizeof(Real));
+    double box[6];
+    for (int i = 0; i < 6; ++i) box[i] = o.region[i];
+    int64_t n[3] = {(int64_t)o.gridSize[0], (int64_t)o.gridSize[1], (int64_t)o.gridSize[2]};
+    dtfe_outputs out{};
+    out.density = d_den.p;
+    out.velocity = d_v.p;
+    unsigned bits = (o.aField.density ? DTFE_FIELD_DENSITY : 0u) | (o.aField.velocity ? DTFE_FIELD_VELOCITY : 0u);
+    ctx.check(dtfe_hip_deposit(ctx.h, scheme, (int64_t)np, d_pos.as<double>(), kDtype, d_w.p, d_vel.p, box, n,
+                               (double)o.averageDensity, bits, &out),
+              scheme == 0 ? "CIC deposit" : "TSC deposit");
+    ctx.check(dtfe_hip_synchronize(ctx.h), "deposit");
+    if (o.aField.density) download(aQ->density, d_den, ncell);
+    if (o.aField.velocity) download(aQ->velocity, d_v, ncell);
+}
+
+// interpolate (src/DTFE.cpp:263-281)
+void interpolate(std::vector<Particle_data> *p, std::vector<Sample_point> &samples, User_options &o, Quantities *uQ,
+                 Quantities *aQ) {
+    if (o.DTFE) dtfe_interpolation(p, samples, o, uQ, aQ);
+    else if (o.CIC) deposit(p, o, aQ, 0);
+    else if (o.TSC) deposit(p, o, aQ, 1);
+    else if (o.NGP) fatal("the NGP method is not provided by the MI355X drop-in");
+    else if (o.SPH) fatal("the SPH method is not provided by the MI355X drop-in");
+    else fatal("unknown interpolation method in function 'interpolate'");
+}
+
+}  // namespace
+
+// DTFE (src/DTFE.cpp:76-251)
+void DTFE(std::vector<Particle_data> *allParticles, std::vector<Sample_point> &samples, User_options &userOptions,
+          Quantities *uQuantities, Quantities *aQuantities) {
+    if (userOptions.poisson != 0) fatal("'--poisson' particle generation is not provided by the MI355X drop-in");
+    if (userOptions.randomSample >= Real(0.))
+        fatal("'--randomSample' subsampling is not provided by the MI355X drop-in");
+    if (userOptions.CIC || userOptions.TSC || userOptions.NGP || userOptions.SPH) userOptions.DTFE = false;
+    std::vector<Particle_data> *pp = allParticles;
+    userOptions.updateEntries(pp->size(), !samples.empty());
+    if (userOptions.averageDensity < 0.) userOptions.averageDensity = averageDensity(*pp, userOptions);
+
+    // the derived velocity fields are produced in the same pass as the gradient
+    User_options t = userOptions;
+    std::vector<Particle_data> regionParts, padded;
+    if (userOptions.regionOn) {
+        for (int i = 0; i < NO_DIM; ++i) {
+            if (t.periodic) {
+                if (t.region[2 * i] < 2 * t.boxCoordinates[2 * i] - t.boxCoordinates[2 * i + 1] ||
+                    t.region[2 * i + 1] > 2 * t.boxCoordinates[2 * i + 1] - t.boxCoordinates[2 * i])
+                    fatal("the region must lie within one box length of the data box");
+            } else if (t.region[2 * i] < t.boxCoordinates[2 * i] || t.region[2 * i + 1] > t.boxCoordinates[2 * i + 1]) {
+                fatal("the region must fit inside the data box for non-periodic data");
+            }
+        }
+        t.paddedBox = t.region;
+        t.paddedBox.addPadding(t.paddingLength);
+        findParticlesInBox(*pp, &regionParts, t);
+        t.periodic = false;
+        t.updateFullBox(t.region);
+        pp->clear();
+        pp = &regionParts;
+    }
+    if (userOptions.partitionOn) fatal("'--partition' is not provided by the MI355X drop-in (one GPU partition)");
+    if (!userOptions.regionOn) {
+        t.region = t.boxCoordinates;
+        t.paddedBox = t.region;
+        t.paddedBox.addPadding(t.paddingLength);
+        if (userOptions.periodic) {
+            findParticlesInBox(*pp, &padded, t);
+            pp->clear();
+            pp = &padded;
+        }
+    }
+    t.periodic = false;
+    interpolate(pp, samples, t, uQuantities, aQuantities);
+    allParticles->clear();
+}

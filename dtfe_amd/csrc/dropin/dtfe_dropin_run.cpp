@@ -1,0 +1,124 @@
+// Command-line driver of the DTFE() drop-in: reads particles and options
+// from files, calls DTFE(), writes every non-empty output field.
+//
+//   dtfe_dropin_run <options.txt> <particles.bin> [<samples.bin>] <out_prefix>
+//
+// options.txt: "key value..." lines (box, grid, periodic, fields_u, fields_a,
+// method, points, seed, cone, origin, region, CIC, TSC, density0);
+// particles.bin: n x 7 doubles (x, y, z, weight, vx, vy, vz);
+// samples.bin: m x 6 doubles (x, y, z, dx, dy, dz).
+// Each output field F of uQuantities/aQuantities goes to <out_prefix>.{u,a}.F
+// as raw Real values.
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "DTFE.h"
+
+static std::vector<double> read_doubles(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    std::vector<double> v;
+    double x;
+    while (f.read(reinterpret_cast<char *>(&x), sizeof(double))) v.push_back(x);
+    return v;
+}
+
+static void set_fields(Field &f, const std::string &list) {
+    std::stringstream ss(list);
+    std::string s;
+    while (std::getline(ss, s, ',')) {
+        if (s == "density") f.density = true;
+        else if (s == "velocity") f.velocity = true;
+        else if (s == "gradient") f.velocity_gradient = true;
+        else if (s == "divergence") f.velocity_divergence = true;
+        else if (s == "shear") f.velocity_shear = true;
+        else if (s == "vorticity") f.velocity_vorticity = true;
+        else if (s == "velocity_std") f.velocity_std = true;
+        else if (s == "scalar") f.scalar = true;
+        else if (s == "scalar_gradient") f.scalar_gradient = true;
+    }
+}
+
+template <typename T>
+static void dump(const std::string &path, const std::vector<T> &v) {
+    if (v.empty()) return;
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(T));
+}
+
+static void dump_all(const std::string &prefix, const Quantities &q) {
+    dump(prefix + ".density", q.density);
+    dump(prefix + ".velocity", q.velocity);
+    dump(prefix + ".velocity_gradient", q.velocity_gradient);
+    dump(prefix + ".velocity_divergence", q.velocity_divergence);
+    dump(prefix + ".velocity_shear", q.velocity_shear);
+    dump(prefix + ".velocity_vorticity", q.velocity_vorticity);
+    dump(prefix + ".velocity_std", q.velocity_std);
+    dump(prefix + ".scalar", q.scalar);
+    dump(prefix + ".scalar_gradient", q.scalar_gradient);
+}
+
+int main(int argc, char **argv) {
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s options.txt particles.bin [samples.bin] out_prefix\n", argv[0]);
+        return 2;
+    }
+    User_options o;
+    o.verboseLevel = 0;
+    std::ifstream opt(argv[1]);
+    std::string line;
+    while (std::getline(opt, line)) {
+        std::stringstream ss(line);
+        std::string k;
+        ss >> k;
+        if (k == "box") { for (int i = 0; i < 6; ++i) ss >> o.boxCoordinates[i]; }
+        else if (k == "grid") { o.gridSize.assign(3, 0); for (int i = 0; i < 3; ++i) ss >> o.gridSize[i]; }
+        else if (k == "periodic") { int v; ss >> v; o.periodic = v != 0; }
+        else if (k == "fields_u") { std::string s; ss >> s; set_fields(o.uField, s); }
+        else if (k == "fields_a") { std::string s; ss >> s; set_fields(o.aField, s); }
+        else if (k == "method") ss >> o.method;
+        else if (k == "points") { ss >> o.noPoints; o.noPointsOn = true; }
+        else if (k == "seed") ss >> o.randomSeed;
+        else if (k == "cone") { o.redshiftConeOn = true; for (int i = 0; i < 6; ++i) ss >> o.redshiftCone[i]; }
+        else if (k == "origin") { o.originPosition.assign(3, 0); for (int i = 0; i < 3; ++i) ss >> o.originPosition[i]; }
+        else if (k == "region") { o.regionOn = true; for (int i = 0; i < 6; ++i) ss >> o.region[i]; }
+        else if (k == "CIC") o.CIC = true;
+        else if (k == "TSC") o.TSC = true;
+        else if (k == "density0") ss >> o.averageDensity;
+        else if (k == "value_formula") { int v; ss >> v; o.referenceValueFormula = v != 0; }
+    }
+    std::vector<double> pd = read_doubles(argv[2]);
+    std::vector<Particle_data> parts(pd.size() / 7);
+    for (size_t i = 0; i < parts.size(); ++i) {
+        for (int k = 0; k < 3; ++k) parts[i].pos[k] = (Real)pd[7 * i + k];
+        parts[i].weight() = (Real)pd[7 * i + 3];
+        for (int k = 0; k < 3; ++k) parts[i].velocity()[k] = (Real)pd[7 * i + 4 + k];
+    }
+    std::vector<Sample_point> samples;
+    std::string prefix = argv[argc - 1];
+    if (argc >= 5) {
+        std::vector<double> sd = read_doubles(argv[3]);
+        samples.resize(sd.size() / 6);
+        for (size_t i = 0; i < samples.size(); ++i)
+            for (int k = 0; k < 3; ++k) {
+                samples[i].pos[k] = (Real)sd[6 * i + k];
+                samples[i]._delta[k] = (Real)sd[6 * i + 3 + k];
+            }
+    }
+    dtfe_dropin::throw_on_error(true);
+    Quantities uq, aq;
+    try {
+        DTFE(&parts, samples, o, &uq, &aq);
+    } catch (const dtfe_dropin::Error &e) {
+        std::fprintf(stderr, "DTFE error: %s\n", e.what());
+        return 1;
+    }
+    dump_all(prefix + ".u", uq);
+    dump_all(prefix + ".a", aq);
+    std::printf("ok particles_left=%zu averageDensity=%.17g tessellation_s=%.3f\n", parts.size(),
+                (double)o.averageDensity, dtfe_dropin::last_tessellation_seconds());
+    return 0;
+}

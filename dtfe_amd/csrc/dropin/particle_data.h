@@ -1,0 +1,59 @@
+// Particle and sample-point types with the interface of the reference's
+// src/particle_data.h:47-122 (weight, density, velocity, scalars, position;
+// Sample_point with position and cell size delta).
+#ifndef DTFE_DROPIN_PARTICLE_DATA_H
+#define DTFE_DROPIN_PARTICLE_DATA_H
+
+#include "Pvector.h"
+#include "define.h"
+
+class Data_structure {
+  protected:
+    Real _weight;
+    Real _density;
+    Pvector<Real, noVelComp> _velocity;
+    Pvector<Real, noScalarComp> _scalar;
+
+  public:
+    Data_structure() : _weight(Real(1.)), _density(Real(0.)) {}
+    Real &weight() { return _weight; }
+    void setWeight(Real w) { _weight = w; }
+    Real &density() { return _density; }
+    void setDensity(Real d) { _density = d; }
+    Pvector<Real, noVelComp> &velocity() { return _velocity; }
+    Real &velocity(int i) { return _velocity[i]; }
+    void setVelocity(Real *v) {
+        for (size_t j = 0; j < noVelComp; ++j) _velocity[j] = v[j];
+    }
+    Pvector<Real, noScalarComp> &scalar() { return _scalar; }
+    Real &scalar(int i) { return _scalar[i]; }
+    void setScalar(Real *s) {
+        for (size_t j = 0; j < noScalarComp; ++j) _scalar[j] = s[j];
+    }
+};
+
+struct Particle_data : public Data_structure {
+    Pvector<Real, NO_DIM> pos;
+    Pvector<Real, NO_DIM> &position() { return pos; }
+    Real &position(int i) { return pos[i]; }
+    void setPosition(Real *p) {
+        for (size_t j = 0; j < NO_DIM; ++j) pos[j] = p[j];
+    }
+};
+
+struct Sample_point {
+    Pvector<Real, NO_DIM> pos;
+    Pvector<Real, NO_DIM> _delta;
+    Pvector<Real, NO_DIM> &position() { return pos; }
+    Real &position(int i) { return pos[i]; }
+    void setPosition(Real *p) {
+        for (size_t j = 0; j < NO_DIM; ++j) pos[j] = p[j];
+    }
+    Pvector<Real, NO_DIM> &delta() { return _delta; }
+    Real &delta(int i) { return _delta[i]; }
+    void setDelta(Real *p) {
+        for (size_t j = 0; j < NO_DIM; ++j) _delta[j] = p[j];
+    }
+};
+
+#endif

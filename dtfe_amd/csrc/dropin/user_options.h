@@ -1,0 +1,138 @@
+// Options of the DTFE drop-in, with the member names and defaults of the
+// reference's Field / Box / User_options (src/user_options.h:48-199,
+// src/box.h:43-181, defaults src/user_options.cc:40-91). Command-line and
+// config-file parsing (Boost.ProgramOptions) is not part of the drop-in.
+#ifndef DTFE_DROPIN_USER_OPTIONS_H
+#define DTFE_DROPIN_USER_OPTIONS_H
+
+#include <string>
+#include <vector>
+
+#include "define.h"
+
+struct Field {
+    bool triangulation, density, velocity, velocity_gradient, velocity_divergence, velocity_shear,
+        velocity_vorticity, velocity_std, scalar, scalar_gradient;
+    Field()
+        : triangulation(false), density(false), velocity(false), velocity_gradient(false),
+          velocity_divergence(false), velocity_shear(false), velocity_vorticity(false), velocity_std(false),
+          scalar(false), scalar_gradient(false) {}
+    bool selected() const {
+        return density || velocity || velocity_gradient || velocity_divergence || velocity_shear ||
+               velocity_vorticity || velocity_std || scalar || scalar_gradient;
+    }
+    bool selectedVelocityDerivatives() const { return velocity_divergence || velocity_shear || velocity_vorticity; }
+    void deselectVelocityDerivatives() { velocity_divergence = velocity_shear = velocity_vorticity = false; }
+    bool selectedVelocity() const {
+        return velocity || velocity_gradient || velocity_divergence || velocity_shear || velocity_vorticity ||
+               velocity_std;
+    }
+    void deselectVelocity() {
+        velocity = velocity_gradient = velocity_divergence = velocity_shear = velocity_vorticity = velocity_std =
+            false;
+    }
+    bool selectedScalar() const { return scalar || scalar_gradient; }
+    void deselectScalar() { scalar = scalar_gradient = false; }
+};
+
+struct Box {
+    std::vector<Real> coords;  // xMin, xMax, yMin, yMax, zMin, zMax
+    Box() { coords.assign(2 * NO_DIM, Real(0.)); }
+    template <typename P>
+    bool isParticleInBox(P &p) const {
+        return p.pos[0] >= coords[0] && p.pos[0] <= coords[1] && p.pos[1] >= coords[2] && p.pos[1] <= coords[3] &&
+               p.pos[2] >= coords[4] && p.pos[2] <= coords[5];
+    }
+    template <typename Pt>
+    bool isPointInBox(Pt &p) const {
+        return p[0] >= coords[0] && p[0] <= coords[1] && p[1] >= coords[2] && p[1] <= coords[3] &&
+               p[2] >= coords[4] && p[2] <= coords[5];
+    }
+    void translate(const Real *offset) {
+        for (size_t i = 0; i < NO_DIM; ++i) {
+            coords[2 * i] += offset[i];
+            coords[2 * i + 1] += offset[i];
+        }
+    }
+    template <typename T>
+    void addPadding(T padding) {
+        for (size_t i = 0; i < NO_DIM; ++i) {
+            coords[2 * i] -= padding[2 * i];
+            coords[2 * i + 1] += padding[2 * i + 1];
+        }
+    }
+    // The reference returns 35000^3 for every box (src/box.h:141-148); the
+    // average density and the CIC/TSC normalisation depend on it.
+    Real volume() const { return Real(35000.) * Real(35000.) * Real(35000.); }
+    Real &operator[](size_t i) { return coords[i]; }
+    const Real &operator[](size_t i) const { return coords[i]; }
+    void assign(Real v) { coords.assign(2 * NO_DIM, v); }
+    size_t size() const { return coords.size(); }
+    bool isNullBox() const {
+        for (Real c : coords)
+            if (c != Real(0.)) return false;
+        return true;
+    }
+};
+
+struct User_options {
+    std::string inputFilename, outputFilename;
+    std::vector<size_t> gridSize;
+    bool periodic;
+    Box boxCoordinates;
+    bool userGivenBoxCoordinates;
+    int inputFileType, outputFileType;
+    Field uField, aField;
+    bool partitionOn;
+    std::vector<size_t> partition;
+    int partNo;
+    bool paddingOn, paddingMpcOn;
+    Real paddingParticles;
+    Box paddingLength;
+    bool testPaddedBoundaries;
+    bool regionOn, regionMpcOn;
+    Box region;
+    int method;
+    int noPoints;
+    bool noPointsOn;
+    Real averageDensity;
+    size_t randomSeed;
+    std::vector<Real> originPosition;
+    bool redshiftConeOn;
+    Box redshiftCone;
+    bool NGP, CIC, TSC, SPH;
+    int SPH_neighbors;
+    Real MpcValue;
+    bool extensive;
+    int verboseLevel;
+    Real randomSample;
+    size_t poisson;
+    // set at run time
+    Box paddedBox;
+    std::vector<Real> fullBoxOffset, fullBoxLength;
+    bool DTFE;
+    bool userDefinedSampling;
+    int noProcessors, threadId;
+    Real totalTime;
+    // drop-in extensions: GPU device and the reference's velocity/scalar value formula
+    int device;
+    bool referenceValueFormula;
+
+    User_options()
+        : periodic(false), userGivenBoxCoordinates(false), inputFileType(0), outputFileType(0), partitionOn(false),
+          partNo(-1), paddingOn(false), paddingMpcOn(false), paddingParticles(Real(5.)), testPaddedBoundaries(false),
+          regionOn(false), regionMpcOn(false), method(1), noPoints(100), noPointsOn(false), averageDensity(Real(-1.)),
+          randomSeed(0), redshiftConeOn(false), NGP(false), CIC(false), TSC(false), SPH(false), SPH_neighbors(0),
+          MpcValue(Real(MPC_UNIT)), extensive(false), verboseLevel(3), randomSample(Real(-1.)), poisson(0),
+          DTFE(true), userDefinedSampling(false), noProcessors(1), threadId(0), totalTime(Real(0.)), device(0),
+          referenceValueFormula(true) {
+        boxCoordinates.assign(0.);
+        paddedBox.assign(0.);
+    }
+
+    void updateFullBox(Box &newBox);
+    void updateEntries(size_t noTotalParticles, bool userSampling);
+    void updatePadding(size_t noParticles);
+};
+
+#endif

@@ -33,6 +33,22 @@
 namespace dtfe {
 
 constexpr int kBlock = 256;
+
+// Occupancy floor of the sampling kernels (waves per SIMD); 0 lets the
+// compiler choose the register budget.
+#ifndef DTFE_MIN_WAVES
+#define DTFE_MIN_WAVES 0
+#endif
+// Seed-grid buckets per mesh vertex (walks start from the seed of the bucket
+// holding the point).
+#ifndef DTFE_SEED_BUCKETS_PER_VERTEX
+#define DTFE_SEED_BUCKETS_PER_VERTEX 0.5
+#endif
+#if DTFE_MIN_WAVES > 0
+#define DTFE_SAMPLE_BOUNDS __launch_bounds__(kBlock, DTFE_MIN_WAVES)
+#else
+#define DTFE_SAMPLE_BOUNDS __launch_bounds__(kBlock)
+#endif
 constexpr int kMaxScalars = 4;
 constexpr int kMaxWalkSteps = 1 << 20;
 
@@ -95,6 +111,7 @@ __device__ __forceinline__ int face_sos(int i, const d3 X[4], d3 p) {
 }
 
 struct Located {
+    int steps;    // tetrahedra visited
     int t;        // cell, -1 outside the hull, -2 walk did not terminate
     int4 v;       // vertex ids of t
     d3 X[4];      // vertex positions of t
@@ -128,6 +145,7 @@ __device__ Located locate(const double *__restrict__ pos, const int4 *__restrict
     Located L;
     int prev = -1;
     for (int step = 0; step < kMaxWalkSteps; ++step) {
+        L.steps = step + 1;
         int4 c = cells[t];
         int4 nb4 = nbrs[t];
         L.X[0] = load_pos(pos, c.x);
@@ -277,7 +295,9 @@ __device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexF
         }
         acc.d += val;
     }
-    if (VEL && (fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_STD))) {
+    constexpr unsigned kGradUsers = DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE |
+                                    DTFE_FIELD_VELOCITY_SHEAR | DTFE_FIELD_VELOCITY_VORTICITY;
+    if (VEL && (fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_STD | kGradUsers))) {
         F vb[4][3];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -319,7 +339,7 @@ __device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexF
                 acc.wm2 += m2;
             }
         }
-        if (fields & DTFE_FIELD_VELOCITY_GRADIENT)
+        if (fields & kGradUsers)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -456,6 +476,11 @@ struct SampleArgs {
     int64_t *defer_rows;            // rows the exact kernel redoes
     uint32_t *defer_words;          // their MC words, 3M per row
     int defer_cap;
+    // adjoint: upstream gradients per row and per-vertex accumulators
+    const void *g_density, *g_velocity, *g_grad, *g_div, *g_shear, *g_vort;
+    void *d_rho;   // [nv] dL/d rho_v
+    void *d_vel;   // [nv][3]
+    void *d_pos;   // [nv][3]
 };
 
 __device__ __forceinline__ int seed_cell(const SeedGrid &sg, d3 p) {
@@ -549,6 +574,9 @@ __device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, co
     int start = seed_cell(a.sg, centre);
     Located Lc = locate<EXACT>(a.pos, a.cells, a.nbrs, centre, start);
     if (!EXACT && Lc.t == -3) return false;
+#ifdef DTFE_COUNT_STEPS
+    int sample_steps = 0;
+#endif
     if (Lc.t == -2) {
         atomicAdd(a.status, 1);
         Lc.t = -1;
@@ -579,6 +607,9 @@ __device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, co
         }
         Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, p, hint);
         if (!EXACT && L.t == -3) return false;
+#ifdef DTFE_COUNT_STEPS
+        sample_steps += L.steps;
+#endif
         if (L.t == -2) {
             atomicAdd(a.status, 1);
             L.t = -1;
@@ -591,6 +622,10 @@ __device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, co
         add_sample<F, VEL, NS>(L, p, a.vf, a.fields, value_bug, acc, want_std);
     }
     if (a.out.cell) a.out.cell[row] = Lc.t;
+#ifdef DTFE_COUNT_STEPS
+    atomicAdd((unsigned long long *)(a.status + 4), (unsigned long long)Lc.steps);
+    atomicAdd((unsigned long long *)(a.status + 6), (unsigned long long)sample_steps);
+#endif
     F scale;
     if (a.flags & DTFE_AVERAGE_FINITE_ONLY) scale = finite > 0 ? (F)1 / (F)finite : (F)0;
     else scale = (F)1 / (F)a.M;
@@ -600,6 +635,242 @@ __device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, co
         stf<F>(a.out.velstd, row, sqrt(var));
     }
     return true;
+}
+
+// ----------------------------------------------------------------------------
+// adjoint of the sampling (reverse mode, fixed tessellation)
+// ----------------------------------------------------------------------------
+// With lambda the barycentric coordinates of a fixed sample point p in
+// tetrahedron t (vertices X_k, values f_k):
+//   d lambda_i / d X_k = -lambda_k grad(lambda_i),
+//   d grad(lambda_i) / d X_k = -grad(lambda_k) (x) grad(lambda_i),
+// hence d rho(p)/d X_k = -lambda_k grad(rho) and
+// d G_ab / d X_k = -grad(lambda_k)_b grad(v_a), G_ab = d v_a / d x_b.
+template <typename F>
+__device__ __forceinline__ void atomic_add_f(void *base, int64_t i, F v) {
+    atomicAdd(reinterpret_cast<F *>(base) + i, v);
+}
+
+template <typename F>
+struct RowGrad {
+    F gr;       // dL/d density
+    F gv[3];    // dL/d velocity
+    F gG[9];    // dL/d velocity gradient (derived fields folded in)
+    bool any_g;
+};
+
+template <typename F, bool VEL>
+__device__ __forceinline__ RowGrad<F> row_upstream(const SampleArgs &a, int64_t row) {
+    RowGrad<F> g;
+    g.gr = a.g_density ? ldf<F>(a.g_density, row) : (F)0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g.gv[c] = (VEL && a.g_velocity) ? ldf<F>(a.g_velocity, 3 * row + c) : (F)0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) g.gG[c] = (VEL && a.g_grad) ? ldf<F>(a.g_grad, 9 * row + c) : (F)0;
+    if (VEL && a.g_div) {
+        F d = ldf<F>(a.g_div, row);
+        g.gG[0] += d;
+        g.gG[4] += d;
+        g.gG[8] += d;
+    }
+    if (VEL && a.g_vort) {
+        F w0 = ldf<F>(a.g_vort, 3 * row), w1 = ldf<F>(a.g_vort, 3 * row + 1), w2 = ldf<F>(a.g_vort, 3 * row + 2);
+        const F h = (F)0.5;
+        g.gG[1] += h * w0; g.gG[3] -= h * w0;
+        g.gG[2] += h * w1; g.gG[6] -= h * w1;
+        g.gG[5] += h * w2; g.gG[7] -= h * w2;
+    }
+    if (VEL && a.g_shear) {
+        F s0 = ldf<F>(a.g_shear, 5 * row), s1 = ldf<F>(a.g_shear, 5 * row + 1), s2 = ldf<F>(a.g_shear, 5 * row + 2),
+          s3 = ldf<F>(a.g_shear, 5 * row + 3), s4 = ldf<F>(a.g_shear, 5 * row + 4);
+        const F t = (F)1 / (F)3, h = (F)0.5;
+        g.gG[0] += 2 * t * s0 - t * s3;
+        g.gG[4] += 2 * t * s3 - t * s0;
+        g.gG[8] -= t * (s0 + s3);
+        g.gG[1] += h * s1; g.gG[3] += h * s1;
+        g.gG[2] += h * s2; g.gG[6] += h * s2;
+        g.gG[5] += h * s4; g.gG[7] += h * s4;
+    }
+    bool any = g.gr != (F)0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) any = any || g.gv[c] != (F)0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) any = any || g.gG[c] != (F)0;
+    g.any_g = any;
+    return g;
+}
+
+// Scatters one sample's contribution (weight s) to the vertex accumulators.
+template <typename F, bool VEL>
+__device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Located &L, d3 p, const RowGrad<F> &g,
+                                               F s, bool value_bug) {
+    const int id[4] = {L.v.x, L.v.y, L.v.z, L.v.w};
+    double Ai[3][3];
+    bool ok = ref_inverse(L.X, Ai);
+    if (!ok) {
+        // the reference's degenerate rule: every field is vertex 0's value
+        if (g.gr != (F)0) atomic_add_f<F>(a.d_rho, id[0], s * g.gr);
+        if (VEL && a.d_vel)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (g.gv[c] != (F)0) atomic_add_f<F>(a.d_vel, 3 * (int64_t)id[0] + c, s * g.gv[c]);
+        return;
+    }
+    // grad(lambda_i): column i-1 of A^-1 for i >= 1, minus their sum for i = 0
+    F gl[4][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        gl[1][b] = (F)Ai[b][0];
+        gl[2][b] = (F)Ai[b][1];
+        gl[3][b] = (F)Ai[b][2];
+        gl[0][b] = -(gl[1][b] + gl[2][b] + gl[3][b]);
+    }
+    double osum = (L.O[0] + L.O[1]) + (L.O[2] + L.O[3]);
+    F lam[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) lam[i] = (F)(L.O[i] / osum);
+    F dX[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dX[k][0] = dX[k][1] = dX[k][2] = (F)0;
+    if (g.gr != (F)0) {
+        F r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = ldf<F>(a.vf.rho, id[i]);
+        F grho[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) grho[b] = r[0] * gl[0][b] + r[1] * gl[1][b] + r[2] * gl[2][b] + r[3] * gl[3][b];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            atomic_add_f<F>(a.d_rho, id[i], s * g.gr * lam[i]);
+#pragma unroll
+            for (int b = 0; b < 3; ++b) dX[i][b] -= s * g.gr * lam[i] * grho[b];
+        }
+    }
+    if (VEL) {
+        bool gv_any = g.gv[0] != (F)0 || g.gv[1] != (F)0 || g.gv[2] != (F)0;
+        bool gG_any = false;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) gG_any = gG_any || g.gG[c] != (F)0;
+        if (gv_any || gG_any) {
+            F v[4][3];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[i][c] = ldf<F>(a.vf.vel, 3 * (int64_t)id[i] + c);
+            F G[3][3];  // G[a][b] = d v_a / d x_b
+#pragma unroll
+            for (int ca = 0; ca < 3; ++ca)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                    G[ca][b] = v[0][ca] * gl[0][b] + v[1][ca] * gl[1][b] + v[2][ca] * gl[2][b] + v[3][ca] * gl[3][b];
+            F dz = (F)(p.z - L.X[0].z);
+            // dL/dv_i
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int ca = 0; ca < 3; ++ca) {
+                    F acc = 0;
+                    if (gv_any) {
+                        if (value_bug) acc += g.gv[ca] * ((i == 0 ? (F)1 : (F)0) + gl[i][2] * dz);
+                        else acc += g.gv[ca] * lam[i];
+                    }
+                    if (gG_any) acc += g.gG[3 * ca] * gl[i][0] + g.gG[3 * ca + 1] * gl[i][1] + g.gG[3 * ca + 2] * gl[i][2];
+                    if (acc != (F)0 && a.d_vel) atomic_add_f<F>(a.d_vel, 3 * (int64_t)id[i] + ca, s * acc);
+                }
+            // dL/dX_k
+            F Gtgv[3];  // G^T gv
+#pragma unroll
+            for (int b = 0; b < 3; ++b) Gtgv[b] = G[0][b] * g.gv[0] + G[1][b] * g.gv[1] + G[2][b] * g.gv[2];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (gv_any) {
+                    if (value_bug) {
+                        // v_a = v_0a + G_az (p_z - x0_z)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) dX[k][b] -= s * dz * gl[k][2] * Gtgv[b];
+                    } else {
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) dX[k][b] -= s * lam[k] * Gtgv[b];
+                    }
+                }
+                if (gG_any) {
+                    F q[3];  // gG grad(lambda_k)
+#pragma unroll
+                    for (int ca = 0; ca < 3; ++ca)
+                        q[ca] = g.gG[3 * ca] * gl[k][0] + g.gG[3 * ca + 1] * gl[k][1] + g.gG[3 * ca + 2] * gl[k][2];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) dX[k][c] -= s * (G[0][c] * q[0] + G[1][c] * q[1] + G[2][c] * q[2]);
+                }
+            }
+            if (gv_any && value_bug) {
+                // the x0_z of the reference's formula moves with vertex 0
+                dX[0][2] -= s * (g.gv[0] * G[0][2] + g.gv[1] * G[1][2] + g.gv[2] * G[2][2]);
+            }
+        }
+    }
+    if (a.d_pos)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                if (dX[k][b] != (F)0) atomic_add_f<F>(a.d_pos, 3 * (int64_t)id[k] + b, dX[k][b]);
+}
+
+// Backward of one row. Returns false (nothing scattered) when a sign could
+// not be certified; the exact kernel then redoes the row.
+template <typename F, bool VEL, bool EXACT, typename Words>
+__device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, const Words &words) {
+    const bool value_bug = (a.flags & DTFE_COMPAT_VALUE_BUG) != 0;
+    RowGrad<F> g = row_upstream<F, VEL>(a, row);
+    if (!g.any_g) return true;
+    d3 centre = row_centre(a, row);
+    int start = seed_cell(a.sg, centre);
+    Located Lc = locate<EXACT>(a.pos, a.cells, a.nbrs, centre, start);
+    if (!EXACT && Lc.t == -3) return false;
+    if (Lc.t == -2) {
+        atomicAdd(a.status, 1);
+        Lc.t = -1;
+    }
+    if (a.method == DTFE_METHOD_CENTRE) {
+        if (Lc.t >= 0) scatter_sample<F, VEL>(a, Lc, centre, g, (F)1, value_bug);
+        return true;
+    }
+    int hint = Lc.t >= 0 ? Lc.t : start;
+    auto sample_point = [&](int w) {
+        if (a.method == DTFE_METHOD_EQUI) {
+            const double *e = a.equi + 3 * w;
+            return d3{add_rn(centre.x, e[0]), add_rn(centre.y, e[1]), add_rn(centre.z, e[2])};
+        }
+        int64_t word = 3 * ((int64_t)a.M * row + w);
+        double u[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) u[q] = mt_uniform(mt_temper(words(word + q)));
+        return mc_point(a, row, centre, u);
+    };
+    if (!EXACT) {
+        // make sure every walk of the row is certified before scattering anything
+        for (int w = 0; w < a.M; ++w) {
+            Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, sample_point(w), hint);
+            if (L.t == -3) return false;
+        }
+    }
+    const F s = (F)1 / (F)a.M;
+    for (int w = 0; w < a.M; ++w) {
+        d3 p = sample_point(w);
+        Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, p, hint);
+        if (L.t == -2) {
+            atomicAdd(a.status, 1);
+            continue;
+        }
+        if (L.t >= 0) scatter_sample<F, VEL>(a, L, p, g, s, value_bug);
+    }
+    return true;
+}
+
+template <typename F, bool VEL, int NS, bool EXACT, bool ADJ, typename Words>
+__device__ __forceinline__ bool run_row(const SampleArgs &a, int64_t row, const Words &words) {
+    if constexpr (ADJ) return adjoint_row<F, VEL, EXACT>(a, row, words);
+    else return process_row<F, VEL, NS, EXACT>(a, row, words);
 }
 
 // Queues a row for the exact kernel, with its Monte-Carlo words.
@@ -653,19 +924,19 @@ __device__ __forceinline__ void mt_generate_block(uint32_t *ring, int64_t q, int
 }
 
 // Rows without random draws: one thread per row.
-template <typename F, bool VEL, int NS>
-__global__ void __launch_bounds__(kBlock) k_sample_plain(SampleArgs a) {
+template <typename F, bool VEL, int NS, bool ADJ>
+__global__ void DTFE_SAMPLE_BOUNDS k_sample_plain(SampleArgs a) {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     NoWords nw;
     for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride)
-        if (!process_row<F, VEL, NS, false>(a, row, nw)) defer_row(a, row, nw);
+        if (!run_row<F, VEL, NS, false, ADJ>(a, row, nw)) defer_row(a, row, nw);
 }
 
 // Monte-Carlo rows: workgroup b owns rows [b*rows_per_block, ...) and
 // regenerates their MT19937 words in an LDS ring of `ring` MT blocks,
 // starting from the plan state of its first block.
-template <typename F, bool VEL, int NS>
-__global__ void __launch_bounds__(kBlock) k_sample_mc(SampleArgs a) {
+template <typename F, bool VEL, int NS, bool ADJ>
+__global__ void DTFE_SAMPLE_BOUNDS k_sample_mc(SampleArgs a) {
     extern __shared__ uint32_t ring[];
     const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
     if (row0 >= a.n_rows) return;
@@ -686,31 +957,31 @@ __global__ void __launch_bounds__(kBlock) k_sample_mc(SampleArgs a) {
         }
         __syncthreads();
         for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
-            if (!process_row<F, VEL, NS, false>(a, row, words)) defer_row(a, row, words);
+            if (!run_row<F, VEL, NS, false, ADJ>(a, row, words)) defer_row(a, row, words);
     }
 }
 
 // Deferred rows, redone with exact signs (symbolic tie-break).
-template <typename F, bool VEL, int NS>
-__global__ void __launch_bounds__(kBlock) k_sample_exact(SampleArgs a, int n_deferred) {
+template <typename F, bool VEL, int NS, bool ADJ>
+__global__ void DTFE_SAMPLE_BOUNDS k_sample_exact(SampleArgs a, int n_deferred) {
     int stride = gridDim.x * blockDim.x;
     const int64_t wpr = 3 * (int64_t)a.M;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_deferred; i += stride) {
         int64_t row = a.defer_rows[i];
         SavedWords words{a.defer_words + (int64_t)i * wpr, wpr * row};
-        process_row<F, VEL, NS, true>(a, row, words);
+        run_row<F, VEL, NS, true, ADJ>(a, row, words);
     }
 }
 
 // Every row with exact signs (used when the deferred queue overflowed).
-template <typename F, bool VEL, int NS>
-__global__ void __launch_bounds__(kBlock) k_sample_exact_all(SampleArgs a) {
+template <typename F, bool VEL, int NS, bool ADJ>
+__global__ void DTFE_SAMPLE_BOUNDS k_sample_exact_all(SampleArgs a) {
     extern __shared__ uint32_t ring[];
     if (a.method != DTFE_METHOD_MC) {
         int64_t stride = (int64_t)gridDim.x * blockDim.x;
         NoWords nw;
         for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride)
-            process_row<F, VEL, NS, true>(a, row, nw);
+            run_row<F, VEL, NS, true, ADJ>(a, row, nw);
         return;
     }
     const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
@@ -731,7 +1002,7 @@ __global__ void __launch_bounds__(kBlock) k_sample_exact_all(SampleArgs a) {
         }
         __syncthreads();
         for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
-            process_row<F, VEL, NS, true>(a, row, words);
+            run_row<F, VEL, NS, true, ADJ>(a, row, words);
     }
 }
 
@@ -799,21 +1070,36 @@ __global__ void k_bbox_partial(const double *__restrict__ pos, int64_t nv, doubl
         for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = s[k][0];
 }
 
-// bbox[0..2] = lo, bbox[3..5] = g / (hi - lo)
+// bbox[0..2] = lo, bbox[3..5] = g / (hi - lo); one workgroup
 __global__ void k_bbox_final(const double *__restrict__ part, int nparts, int gx, int gy, int gz,
                              double *__restrict__ bbox) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double s[6][kBlock];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int b = 0; b < nparts; ++b)
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
         for (int k = 0; k < 3; ++k) {
             lo[k] = fmin(lo[k], part[6 * b + k]);
             hi[k] = fmax(hi[k], part[6 * b + 3 + k]);
         }
-    int g[3] = {gx, gy, gz};
     for (int k = 0; k < 3; ++k) {
-        double ext = hi[k] - lo[k];
-        bbox[k] = lo[k];
-        bbox[3 + k] = ext > 0 ? g[k] / ext : 0.0;
+        s[k][threadIdx.x] = lo[k];
+        s[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 3; ++k) {
+                s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + off]);
+                s[3 + k][threadIdx.x] = fmax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + off]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int g[3] = {gx, gy, gz};
+        for (int k = 0; k < 3; ++k) {
+            double ext = s[3 + k][0] - s[k][0];
+            bbox[k] = s[k][0];
+            bbox[3 + k] = ext > 0 ? g[k] / ext : 0.0;
+        }
     }
 }
 
@@ -862,6 +1148,63 @@ __global__ void k_points_in_box(const double *__restrict__ pts, int64_t n, const
         const double *q = pts + 3 * i;
         flag[i] = (q[0] >= box[0] && q[0] <= box[1] && q[1] >= box[2] && q[1] <= box[3] && q[2] >= box[4] &&
                    q[2] <= box[5]) ? 1 : 0;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// adjoint of the vertex density: rho_v = 4 w_v / (rho_bar W_v), W_v = sum V_t
+// ----------------------------------------------------------------------------
+template <typename F>
+__global__ void k_adj_vertex(const F *__restrict__ w, const F *__restrict__ rho, const double *__restrict__ W,
+                             const uint8_t *__restrict__ hull, const F *__restrict__ d_rho, int64_t nv,
+                             double rho_bar, F *__restrict__ d_w, double *__restrict__ dW,
+                             double *__restrict__ partial) {
+    __shared__ double red[kBlock];
+    double acc = 0.0;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+        double g = (double)d_rho[v];
+        if (hull[v] || !(W[v] > 0.0) || g == 0.0) {
+            dW[v] = 0.0;
+            continue;
+        }
+        double r = (double)rho[v];
+        if (d_w) d_w[v] += (F)(g * 4.0 / (rho_bar * W[v]));
+        dW[v] = -g * r / W[v];
+        acc += -g * r / rho_bar;  // d rho_v / d rho_bar = -rho_v / rho_bar
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// dV_t/dX_1 = (e2 x e3)/6, dV_t/dX_2 = (e3 x e1)/6, dV_t/dX_3 = (e1 x e2)/6,
+// dV_t/dX_0 = -(their sum), with e_k = X_k - X_0.
+template <typename F>
+__global__ void k_adj_tets(const double *__restrict__ pos, const int4 *__restrict__ cells, int64_t nc,
+                           const double *__restrict__ dW, F *__restrict__ d_pos) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
+        int4 c = cells[t];
+        double dV = (dW[c.x] + dW[c.y]) + (dW[c.z] + dW[c.w]);
+        if (dV == 0.0) continue;
+        d3 x0 = load_pos(pos, c.x), x1 = load_pos(pos, c.y), x2 = load_pos(pos, c.z), x3 = load_pos(pos, c.w);
+        d3 e1 = sub(x1, x0), e2 = sub(x2, x0), e3 = sub(x3, x0);
+        d3 g1 = cross(e2, e3), g2 = cross(e3, e1), g3 = cross(e1, e2);
+        const double k = dV / 6.0;
+        d3 g0{-(g1.x + g2.x + g3.x), -(g1.y + g2.y + g3.y), -(g1.z + g2.z + g3.z)};
+        const int id[4] = {c.x, c.y, c.z, c.w};
+        const d3 gg[4] = {g0, g1, g2, g3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            atomicAdd(d_pos + 3 * (int64_t)id[q] + 0, (F)(k * gg[q].x));
+            atomicAdd(d_pos + 3 * (int64_t)id[q] + 1, (F)(k * gg[q].y));
+            atomicAdd(d_pos + 3 * (int64_t)id[q] + 2, (F)(k * gg[q].z));
+        }
     }
 }
 
@@ -995,8 +1338,9 @@ struct dtfe_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf W, hull, seed_a, seed_b, bbox_part, bbox, status, axes, trig, equi, plan, flags, rows, mass, mom,
-        defer_rows, defer_words;
+        defer_rows, defer_words, adj_rho, adj_dW, adj_partial;
     int last_deferred = 0;
+    int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples
     // HIP events around the main kernels of the last calls (on ctx->stream)
     hipEvent_t ev[8] = {};
     bool ev_pending[4] = {false, false, false, false};
@@ -1058,7 +1402,8 @@ int check_mesh(dtfe_ctx *ctx, const dtfe_mesh *m) {
 // Builds the seed grid of the mesh; returns it in `sg`.
 int build_seed_grid(dtfe_ctx *ctx, const dtfe_mesh *m, SeedGrid &sg) {
     int64_t nv = m->n_vertices, nc = m->n_cells;
-    int g = (int)std::max<double>(1.0, std::floor(std::cbrt(std::max<double>(1.0, nv / 2.0))));
+    int g = (int)std::max<double>(
+        1.0, std::floor(std::cbrt(std::max<double>(1.0, nv * (double)DTFE_SEED_BUCKETS_PER_VERTEX))));
     g = std::min(g, 512);
     int64_t nb = (int64_t)g * g * g;
     int parts = std::min(1024, grid_for(nv));
@@ -1069,7 +1414,7 @@ int build_seed_grid(dtfe_ctx *ctx, const dtfe_mesh *m, SeedGrid &sg) {
     if ((rc = ensure(ctx, ctx->seed_b, nb * sizeof(int)))) return rc;
     hipLaunchKernelGGL(k_bbox_partial, dim3(parts), dim3(kBlock), 0, ctx->stream, m->pos, nv,
                        (double *)ctx->bbox_part.p);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, ctx->stream, (const double *)ctx->bbox_part.p, parts, g,
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, ctx->stream, (const double *)ctx->bbox_part.p, parts, g,
                        g, g, (double *)ctx->bbox.p);
     hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, (int *)ctx->seed_a.p, nb, -1);
     sg.seed = (const int *)ctx->seed_a.p;
@@ -1088,38 +1433,268 @@ int build_seed_grid(dtfe_ctx *ctx, const dtfe_mesh *m, SeedGrid &sg) {
 }
 
 // which: 0 = fast kernel, 1 = deferred rows (exact), 2 = every row exact
-template <typename F, bool VEL, int NS>
+template <typename F, bool VEL, int NS, bool ADJ>
 void launch_sample(const SampleArgs &a, int which, int mc, int grid, size_t lds, int n_def, hipStream_t s) {
     if (which == 1)
-        hipLaunchKernelGGL((k_sample_exact<F, VEL, NS>), dim3(grid), dim3(kBlock), 0, s, a, n_def);
+        hipLaunchKernelGGL((k_sample_exact<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), 0, s, a, n_def);
     else if (which == 2)
-        hipLaunchKernelGGL((k_sample_exact_all<F, VEL, NS>), dim3(grid), dim3(kBlock), mc ? lds : 0, s, a);
+        hipLaunchKernelGGL((k_sample_exact_all<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), mc ? lds : 0, s, a);
     else if (mc)
-        hipLaunchKernelGGL((k_sample_mc<F, VEL, NS>), dim3(grid), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((k_sample_mc<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), lds, s, a);
     else
-        hipLaunchKernelGGL((k_sample_plain<F, VEL, NS>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL((k_sample_plain<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 template <typename F, bool VEL>
-void launch_ns(const SampleArgs &a, int ns, int which, int mc, int grid, size_t lds, int n_def, hipStream_t s) {
+void launch_ns(const SampleArgs &a, int ns, bool adj, int which, int mc, int grid, size_t lds, int n_def,
+               hipStream_t s) {
+    if (adj) {
+        launch_sample<F, VEL, 0, true>(a, which, mc, grid, lds, n_def, s);
+        return;
+    }
     switch (ns) {
-        case 0: launch_sample<F, VEL, 0>(a, which, mc, grid, lds, n_def, s); break;
-        case 1: launch_sample<F, VEL, 1>(a, which, mc, grid, lds, n_def, s); break;
-        case 2: launch_sample<F, VEL, 2>(a, which, mc, grid, lds, n_def, s); break;
-        case 3: launch_sample<F, VEL, 3>(a, which, mc, grid, lds, n_def, s); break;
-        default: launch_sample<F, VEL, 4>(a, which, mc, grid, lds, n_def, s); break;
+        case 0: launch_sample<F, VEL, 0, false>(a, which, mc, grid, lds, n_def, s); break;
+        case 1: launch_sample<F, VEL, 1, false>(a, which, mc, grid, lds, n_def, s); break;
+        case 2: launch_sample<F, VEL, 2, false>(a, which, mc, grid, lds, n_def, s); break;
+        case 3: launch_sample<F, VEL, 3, false>(a, which, mc, grid, lds, n_def, s); break;
+        default: launch_sample<F, VEL, 4, false>(a, which, mc, grid, lds, n_def, s); break;
     }
 }
 
-void launch_any(const SampleArgs &a, bool f64, bool vel, int ns, int which, int mc, int grid, size_t lds, int n_def,
-                hipStream_t s) {
+void launch_any(const SampleArgs &a, bool f64, bool vel, int ns, bool adj, int which, int mc, int grid, size_t lds,
+                int n_def, hipStream_t s) {
     if (f64) {
-        if (vel) launch_ns<double, true>(a, ns, which, mc, grid, lds, n_def, s);
-        else launch_ns<double, false>(a, ns, which, mc, grid, lds, n_def, s);
+        if (vel) launch_ns<double, true>(a, ns, adj, which, mc, grid, lds, n_def, s);
+        else launch_ns<double, false>(a, ns, adj, which, mc, grid, lds, n_def, s);
     } else {
-        if (vel) launch_ns<float, true>(a, ns, which, mc, grid, lds, n_def, s);
-        else launch_ns<float, false>(a, ns, which, mc, grid, lds, n_def, s);
+        if (vel) launch_ns<float, true>(a, ns, adj, which, mc, grid, lds, n_def, s);
+        else launch_ns<float, false>(a, ns, adj, which, mc, grid, lds, n_def, s);
     }
+}
+
+// Rows of a sampling request (grid / cone / user points after the region
+// filter); sets a.n_rows and the per-kind coordinate data.
+int setup_rows(dtfe_ctx *ctx, const dtfe_sampling *smp, int method, SampleArgs &a, std::vector<double> &axes,
+               std::vector<double> &trig) {
+    int rc;
+    int64_t n_rows = 0;
+    if (smp->kind == DTFE_SAMPLE_GRID || smp->kind == DTFE_SAMPLE_CONE) {
+        for (int i = 0; i < 3; ++i)
+            if (smp->n[i] < 1) return fail(ctx, DTFE_ERR_ARGUMENT, "grid sizes must be >= 1");
+        if ((double)smp->n[0] * smp->n[1] * smp->n[2] > 4.0e9)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "more than 4e9 grid cells");
+        n_rows = smp->n[0] * smp->n[1] * smp->n[2];
+        double bx[6];
+        for (int i = 0; i < 6; ++i) bx[i] = smp->kind == DTFE_SAMPLE_GRID ? smp->box[i] : smp->cone[i];
+        if (smp->kind == DTFE_SAMPLE_CONE) {
+            // checkAngles (triangulation_miscellaneous.cc:162-199), degrees -> radians with PI = 3.14159265
+            if (bx[2] < 0 || bx[2] > 180 || bx[3] < 0 || bx[3] > 180)
+                return fail(ctx, DTFE_ERR_ARGUMENT, "cone theta limits must be in [0, 180] degrees");
+            if (bx[5] - bx[4] >= 360.0) return fail(ctx, DTFE_ERR_ARGUMENT, "cone psi range must be < 360 degrees");
+            double *ang = bx + 2;
+            ang[2] -= std::floor(ang[2] / 360.) * 360.;
+            ang[3] -= std::floor(ang[3] / 360.) * 360.;
+            if (ang[2] > ang[3]) ang[3] += 360.;
+            const double deg = 3.14159265 / 180.;
+            for (int i = 0; i < 4; ++i) ang[i] *= deg;
+        }
+        for (int i = 0; i < 3; ++i) a.dx[i] = (bx[2 * i + 1] - bx[2 * i]) / (double)smp->n[i];
+        // centres by the reference's running sums (x = x0 + dx/2; x += dx)
+        for (int ax = 0; ax < 3; ++ax) {
+            double x = bx[2 * ax] + a.dx[ax] / 2.;
+            for (int64_t i = 0; i < smp->n[ax]; ++i) {
+                axes.push_back(x);
+                x += a.dx[ax];
+            }
+        }
+        if (smp->kind == DTFE_SAMPLE_CONE) {
+            const double *th = axes.data() + smp->n[0];
+            const double *ps = th + smp->n[1];
+            for (int64_t j = 0; j < smp->n[1]; ++j) trig.push_back(std::sin(th[j]));
+            for (int64_t j = 0; j < smp->n[1]; ++j) trig.push_back(std::cos(th[j]));
+            for (int64_t k = 0; k < smp->n[2]; ++k) trig.push_back(std::sin(ps[k]));
+            for (int64_t k = 0; k < smp->n[2]; ++k) trig.push_back(std::cos(ps[k]));
+            for (int i = 0; i < 3; ++i) a.origin[i] = smp->origin[i];
+        }
+        for (int i = 0; i < 3; ++i) a.n[i] = smp->n[i];
+    } else if (smp->kind == DTFE_SAMPLE_POINTS) {
+        if (smp->n_points < 0) return fail(ctx, DTFE_ERR_ARGUMENT, "negative point count");
+        if (smp->n_points > 0 && !smp->points) return fail(ctx, DTFE_ERR_ARGUMENT, "points is NULL");
+        if (method == DTFE_METHOD_MC && smp->n_points > 0 && !smp->deltas)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "Monte-Carlo user sampling needs deltas");
+        a.n[0] = a.n[1] = a.n[2] = 1;
+        a.pts = smp->points;
+        a.deltas = smp->deltas;
+        // region filter + compaction (host round trip of one flag per point)
+        int64_t np = smp->n_points;
+        const size_t box_off = ((size_t)std::max<int64_t>(np, 1) * sizeof(int) + 255) & ~(size_t)255;
+        if ((rc = ensure(ctx, ctx->flags, box_off + 6 * sizeof(double)))) return rc;
+        double *dbox = (double *)((char *)ctx->flags.p + box_off);
+        HIP_TRY(hipMemcpyAsync(dbox, smp->box, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        std::vector<int> fl(np);
+        if (np > 0) {
+            hipLaunchKernelGGL(k_points_in_box, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, smp->points, np,
+                               dbox, (int *)ctx->flags.p);
+            HIP_TRY(hipMemcpyAsync(fl.data(), ctx->flags.p, np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+        std::vector<int64_t> rows;
+        rows.reserve(np);
+        for (int64_t i = 0; i < np; ++i)
+            if (fl[i]) rows.push_back(i);
+        n_rows = (int64_t)rows.size();
+        if ((rc = ensure(ctx, ctx->rows, std::max<int64_t>(n_rows, 1) * sizeof(int64_t)))) return rc;
+        if (n_rows)
+            HIP_TRY(hipMemcpyAsync(ctx->rows.p, rows.data(), n_rows * sizeof(int64_t), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        a.row_point = (const int64_t *)ctx->rows.p;
+        a.n[0] = n_rows > 0 ? n_rows : 1;
+    } else {
+        return fail(ctx, DTFE_ERR_ARGUMENT, "unknown sampling kind");
+    }
+    a.n_rows = n_rows;
+    return DTFE_OK;
+}
+
+// Coordinate tables, equidistant offsets, seed grid and status words.
+int setup_tables(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_mc *mc, SampleArgs &a,
+                 const std::vector<double> &axes, const std::vector<double> &trig) {
+    int rc;
+    if (!axes.empty()) {
+        if ((rc = ensure(ctx, ctx->axes, axes.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->axes.p, axes.data(), axes.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.ax0 = (const double *)ctx->axes.p;
+        a.ax1 = a.ax0 + a.n[0];
+        a.ax2 = a.ax1 + a.n[1];
+    }
+    if (!trig.empty()) {
+        if ((rc = ensure(ctx, ctx->trig, trig.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->trig.p, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.trig = (const double *)ctx->trig.p;
+    }
+    if (a.method == DTFE_METHOD_EQUI) {
+        int NN = mc->samples;
+        int n3 = (int)std::lround(std::cbrt((double)NN));
+        if (n3 * n3 * n3 != NN)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "equidistant averaging needs a perfect cube of samples");
+        std::vector<double> sp(3 * (size_t)NN);
+        for (int i1 = 0; i1 < n3; ++i1)
+            for (int i2 = 0; i2 < n3; ++i2)
+                for (int i3 = 0; i3 < n3; ++i3) {
+                    int idx = i1 * n3 * n3 + i2 * n3 + i3;
+                    // averaged_interpolation_2.cc:725-740
+                    sp[3 * idx + 2] = (i3 + 0.5) * a.dx[2] / n3 - a.dx[2] / 2.;
+                    sp[3 * idx + 0] = (i1 + 0.5) * a.dx[0] / n3 - a.dx[0] / 2.;
+                    sp[3 * idx + 1] = (i2 + 0.5) * a.dx[1] / n3 - a.dx[1] / 2.;
+                }
+        if ((rc = ensure(ctx, ctx->equi, sp.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(ctx->equi.p, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+        a.equi = (const double *)ctx->equi.p;
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    if ((rc = build_seed_grid(ctx, m, a.sg))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    if ((rc = ensure(ctx, ctx->status, 64))) return rc;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->status.p, (int64_t)16, 0);
+    a.status = (int *)ctx->status.p;
+    return DTFE_OK;
+}
+
+// MC plan, main launch, deferred rows, status check.
+int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool vel, int ns, bool adj) {
+    int rc;
+    const int64_t n_rows = a.n_rows;
+    int grid;
+    size_t lds = 0;
+    int is_mc = a.method == DTFE_METHOD_MC;
+    if (is_mc) {
+        const int64_t wpr = 3 * (int64_t)a.M;
+        int rpr = kBlock;
+        // ring must hold every MT block one round touches plus the previous one
+        while (rpr > 1 && ((wpr * rpr + MT_N - 1) / MT_N + 2) * MT_N * 4 > 48 * 1024) rpr /= 2;
+        a.rows_per_round = rpr;
+        a.ring = (int)((wpr * rpr + MT_N - 1) / MT_N + 2);
+        lds = (size_t)a.ring * MT_N * sizeof(uint32_t);
+        int64_t rounds = std::max<int64_t>(1, std::min<int64_t>(16, n_rows / ((int64_t)rpr * 2048)));
+        a.rows_per_block = rpr * rounds;
+        int64_t nblk = (n_rows + a.rows_per_block - 1) / a.rows_per_block;
+        grid = (int)nblk;
+        uint64_t s = mc->seed;
+        if (mc->part_no >= 0) s += 1000ull * (uint64_t)mc->part_no;
+        uint32_t seed32 = (uint32_t)s;
+        if (!(ctx->plan_valid && ctx->plan_seed == seed32 && ctx->plan_wpr == wpr && ctx->plan_rows == n_rows &&
+              ctx->plan_rpb == a.rows_per_block)) {
+            std::vector<int64_t> first(nblk);
+            for (int64_t b = 0; b < nblk; ++b) first[b] = (wpr * (b * a.rows_per_block)) / MT_N;
+            mt_build_plan(seed32, first, ctx->plan_host);
+            if ((rc = ensure(ctx, ctx->plan, ctx->plan_host.size() * sizeof(uint32_t)))) return rc;
+            HIP_TRY(hipMemcpyAsync(ctx->plan.p, ctx->plan_host.data(), ctx->plan_host.size() * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            ctx->plan_valid = true;
+            ctx->plan_seed = seed32;
+            ctx->plan_wpr = wpr;
+            ctx->plan_rows = n_rows;
+            ctx->plan_rpb = a.rows_per_block;
+        }
+        a.plan = (const uint32_t *)ctx->plan.p;
+    } else {
+        grid = grid_for(n_rows);
+    }
+    // rows whose walk meets a sign the FP64 filter cannot certify are queued
+    // for the exact kernel, with their MC words
+    const int64_t wpr = is_mc ? 3 * (int64_t)a.M : 0;
+    int64_t cap = std::min<int64_t>(n_rows, std::max<int64_t>(1 << 16, n_rows / 16));
+    if ((rc = ensure(ctx, ctx->defer_rows, cap * sizeof(int64_t)))) return rc;
+    if (wpr && (rc = ensure(ctx, ctx->defer_words, cap * wpr * sizeof(uint32_t)))) return rc;
+    a.defer_rows = (int64_t *)ctx->defer_rows.p;
+    a.defer_words = (uint32_t *)ctx->defer_words.p;
+    a.defer_cap = (int)cap;
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    launch_any(a, f64, vel, ns, adj, 0, is_mc, grid, lds, 0, ctx->stream);
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    HIP_TRY(hipGetLastError());
+    int st[8];
+    HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    {
+        int64_t cs, ss;
+        std::memcpy(&cs, st + 4, 8);
+        std::memcpy(&ss, st + 6, 8);
+        ctx->last_walk[0] = cs;
+        ctx->last_walk[1] = ss;
+        ctx->last_walk[2] = n_rows;
+        ctx->last_walk[3] = n_rows * (int64_t)(a.method == DTFE_METHOD_CENTRE ? 0 : a.M);
+    }
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->last_ms[0] = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    ctx->last_ms[2] = ms;
+    ctx->last_ms[1] = 0.0;
+    ctx->last_deferred = st[1];
+    if (st[2]) {
+        // queue overflow: redo everything with exact signs (the adjoint never
+        // scatters a row before all of its walks are certified)
+        launch_any(a, f64, vel, ns, adj, 2, is_mc, grid, lds, 0, ctx->stream);
+    } else if (st[1] > 0) {
+        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+        launch_any(a, f64, vel, ns, adj, 1, is_mc, grid_for(st[1]), 0, st[1], ctx->stream);
+        HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+        ctx->ev_pending[1] = true;
+    }
+    HIP_TRY(hipGetLastError());
+    if (st[1] > 0 || st[2]) {
+        HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    if (st[0] > 0)
+        return fail(ctx, DTFE_ERR_WALK,
+                    std::to_string(st[0]) + " point locations did not terminate: the mesh is not a valid triangulation");
+    return DTFE_OK;
 }
 
 }  // namespace
@@ -1157,7 +1732,8 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_a, &ctx->seed_b, &ctx->bbox_part, &ctx->bbox,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
-                      &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words};
+                      &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
+                      &ctx->adj_partial};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 8; ++i)
@@ -1281,83 +1857,9 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
                     out->scalar_gradient, out->located_cell};
     const size_t esz = vf->dtype == DTFE_F64 ? 8 : 4;
 
-    // ---- rows and their coordinates ------------------------------------------------
-    int64_t n_rows = 0;
-    std::vector<double> axes;
-    std::vector<double> trig;
-    if (smp->kind == DTFE_SAMPLE_GRID || smp->kind == DTFE_SAMPLE_CONE) {
-        for (int i = 0; i < 3; ++i)
-            if (smp->n[i] < 1) return fail(ctx, DTFE_ERR_ARGUMENT, "grid sizes must be >= 1");
-        n_rows = smp->n[0] * smp->n[1] * smp->n[2];
-        double bx[6];
-        for (int i = 0; i < 6; ++i) bx[i] = smp->kind == DTFE_SAMPLE_GRID ? smp->box[i] : smp->cone[i];
-        if (smp->kind == DTFE_SAMPLE_CONE) {
-            // checkAngles (triangulation_miscellaneous.cc:162-199), degrees -> radians with PI = 3.14159265
-            if (bx[2] < 0 || bx[2] > 180 || bx[3] < 0 || bx[3] > 180)
-                return fail(ctx, DTFE_ERR_ARGUMENT, "cone theta limits must be in [0, 180] degrees");
-            if (bx[5] - bx[4] >= 360.0) return fail(ctx, DTFE_ERR_ARGUMENT, "cone psi range must be < 360 degrees");
-            double *ang = bx + 2;
-            ang[2] -= std::floor(ang[2] / 360.) * 360.;
-            ang[3] -= std::floor(ang[3] / 360.) * 360.;
-            if (ang[2] > ang[3]) ang[3] += 360.;
-            const double deg = 3.14159265 / 180.;
-            for (int i = 0; i < 4; ++i) ang[i] *= deg;
-        }
-        for (int i = 0; i < 3; ++i) a.dx[i] = (bx[2 * i + 1] - bx[2 * i]) / (double)smp->n[i];
-        // centres by the reference's running sums (x = x0 + dx/2; x += dx)
-        for (int ax = 0; ax < 3; ++ax) {
-            double x = bx[2 * ax] + a.dx[ax] / 2.;
-            for (int64_t i = 0; i < smp->n[ax]; ++i) {
-                axes.push_back(x);
-                x += a.dx[ax];
-            }
-        }
-        if (smp->kind == DTFE_SAMPLE_CONE) {
-            const double *th = axes.data() + smp->n[0];
-            const double *ps = th + smp->n[1];
-            for (int64_t j = 0; j < smp->n[1]; ++j) trig.push_back(std::sin(th[j]));
-            for (int64_t j = 0; j < smp->n[1]; ++j) trig.push_back(std::cos(th[j]));
-            for (int64_t k = 0; k < smp->n[2]; ++k) trig.push_back(std::sin(ps[k]));
-            for (int64_t k = 0; k < smp->n[2]; ++k) trig.push_back(std::cos(ps[k]));
-            for (int i = 0; i < 3; ++i) a.origin[i] = smp->origin[i];
-        }
-        for (int i = 0; i < 3; ++i) a.n[i] = smp->n[i];
-    } else if (smp->kind == DTFE_SAMPLE_POINTS) {
-        if (smp->n_points < 0) return fail(ctx, DTFE_ERR_ARGUMENT, "negative point count");
-        if (smp->n_points > 0 && !smp->points) return fail(ctx, DTFE_ERR_ARGUMENT, "points is NULL");
-        if (method == DTFE_METHOD_MC && smp->n_points > 0 && !smp->deltas)
-            return fail(ctx, DTFE_ERR_ARGUMENT, "Monte-Carlo user sampling needs deltas");
-        a.n[0] = a.n[1] = a.n[2] = 1;
-        a.pts = smp->points;
-        a.deltas = smp->deltas;
-        // region filter + compaction (host round trip of one flag per point)
-        int64_t np = smp->n_points;
-        const size_t box_off = ((size_t)std::max<int64_t>(np, 1) * sizeof(int) + 255) & ~(size_t)255;
-        if ((rc = ensure(ctx, ctx->flags, box_off + 6 * sizeof(double)))) return rc;
-        double *dbox = (double *)((char *)ctx->flags.p + box_off);
-        HIP_TRY(hipMemcpyAsync(dbox, smp->box, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        std::vector<int> fl(np);
-        if (np > 0) {
-            hipLaunchKernelGGL(k_points_in_box, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, smp->points, np,
-                               dbox, (int *)ctx->flags.p);
-            HIP_TRY(hipMemcpyAsync(fl.data(), ctx->flags.p, np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-        }
-        std::vector<int64_t> rows;
-        rows.reserve(np);
-        for (int64_t i = 0; i < np; ++i)
-            if (fl[i]) rows.push_back(i);
-        n_rows = (int64_t)rows.size();
-        if ((rc = ensure(ctx, ctx->rows, std::max<int64_t>(n_rows, 1) * sizeof(int64_t)))) return rc;
-        if (n_rows)
-            HIP_TRY(hipMemcpyAsync(ctx->rows.p, rows.data(), n_rows * sizeof(int64_t), hipMemcpyHostToDevice,
-                                   ctx->stream));
-        a.row_point = (const int64_t *)ctx->rows.p;
-        a.n[0] = n_rows > 0 ? n_rows : 1;
-    } else {
-        return fail(ctx, DTFE_ERR_ARGUMENT, "unknown sampling kind");
-    }
-    a.n_rows = n_rows;
+    std::vector<double> axes, trig;
+    if ((rc = setup_rows(ctx, smp, method, a, axes, trig))) return rc;
+    const int64_t n_rows = a.n_rows;
     out->n_written = n_rows;
     if (n_rows == 0) return DTFE_OK;
 
@@ -1385,130 +1887,9 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
                                out->located_cell, n_rows, -1);
         return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: fields are zero");
     }
-
-    // ---- tables ------------------------------------------------------------------------
-    if (!axes.empty()) {
-        if ((rc = ensure(ctx, ctx->axes, axes.size() * sizeof(double)))) return rc;
-        HIP_TRY(hipMemcpyAsync(ctx->axes.p, axes.data(), axes.size() * sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
-        a.ax0 = (const double *)ctx->axes.p;
-        a.ax1 = a.ax0 + a.n[0];
-        a.ax2 = a.ax1 + a.n[1];
-    }
-    if (!trig.empty()) {
-        if ((rc = ensure(ctx, ctx->trig, trig.size() * sizeof(double)))) return rc;
-        HIP_TRY(hipMemcpyAsync(ctx->trig.p, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice,
-                               ctx->stream));
-        a.trig = (const double *)ctx->trig.p;
-    }
-    if (method == DTFE_METHOD_EQUI) {
-        int NN = mc->samples;
-        int n3 = (int)std::lround(std::cbrt((double)NN));
-        if (n3 * n3 * n3 != NN) return fail(ctx, DTFE_ERR_ARGUMENT, "equidistant averaging needs a perfect cube of samples");
-        std::vector<double> sp(3 * (size_t)NN);
-        for (int i1 = 0; i1 < n3; ++i1)
-            for (int i2 = 0; i2 < n3; ++i2)
-                for (int i3 = 0; i3 < n3; ++i3) {
-                    int idx = i1 * n3 * n3 + i2 * n3 + i3;
-                    // averaged_interpolation_2.cc:725-740
-                    sp[3 * idx + 2] = (i3 + 0.5) * a.dx[2] / n3 - a.dx[2] / 2.;
-                    sp[3 * idx + 0] = (i1 + 0.5) * a.dx[0] / n3 - a.dx[0] / 2.;
-                    sp[3 * idx + 1] = (i2 + 0.5) * a.dx[1] / n3 - a.dx[1] / 2.;
-                }
-        if ((rc = ensure(ctx, ctx->equi, sp.size() * sizeof(double)))) return rc;
-        HIP_TRY(hipMemcpyAsync(ctx->equi.p, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        a.equi = (const double *)ctx->equi.p;
-    }
-
-    // ---- seed grid + status --------------------------------------------------------------
-    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    if ((rc = build_seed_grid(ctx, m, a.sg))) return rc;
-    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
-    if ((rc = ensure(ctx, ctx->status, 16))) return rc;
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->status.p, (int64_t)4, 0);
-    a.status = (int *)ctx->status.p;
-
-    // ---- launch -------------------------------------------------------------------------------
+    if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
     const int ns = want_scal ? vf->n_scalar : 0;
-    int grid;
-    size_t lds = 0;
-    int is_mc = method == DTFE_METHOD_MC;
-    if (is_mc) {
-        const int64_t wpr = 3 * (int64_t)a.M;
-        int rpr = kBlock;
-        // ring must hold every MT block one round touches plus the previous one
-        while (rpr > 1 && ((wpr * rpr + MT_N - 1) / MT_N + 2) * MT_N * 4 > 48 * 1024) rpr /= 2;
-        a.rows_per_round = rpr;
-        a.ring = (int)((wpr * rpr + MT_N - 1) / MT_N + 2);
-        lds = (size_t)a.ring * MT_N * sizeof(uint32_t);
-        int64_t rounds = std::max<int64_t>(1, std::min<int64_t>(16, n_rows / ((int64_t)rpr * 2048)));
-        a.rows_per_block = rpr * rounds;
-        int64_t nblk = (n_rows + a.rows_per_block - 1) / a.rows_per_block;
-        grid = (int)nblk;
-        uint64_t s = mc->seed;
-        if (mc->part_no >= 0) s += 1000ull * (uint64_t)mc->part_no;
-        uint32_t seed32 = (uint32_t)s;
-        if (!(ctx->plan_valid && ctx->plan_seed == seed32 && ctx->plan_wpr == wpr && ctx->plan_rows == n_rows &&
-              ctx->plan_rpb == a.rows_per_block)) {
-            std::vector<int64_t> first(nblk);
-            for (int64_t b = 0; b < nblk; ++b) first[b] = (wpr * (b * a.rows_per_block)) / MT_N;
-            mt_build_plan(seed32, first, ctx->plan_host);
-            if ((rc = ensure(ctx, ctx->plan, ctx->plan_host.size() * sizeof(uint32_t)))) return rc;
-            HIP_TRY(hipMemcpyAsync(ctx->plan.p, ctx->plan_host.data(), ctx->plan_host.size() * sizeof(uint32_t),
-                                   hipMemcpyHostToDevice, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-            ctx->plan_valid = true;
-            ctx->plan_seed = seed32;
-            ctx->plan_wpr = wpr;
-            ctx->plan_rows = n_rows;
-            ctx->plan_rpb = a.rows_per_block;
-        }
-        a.plan = (const uint32_t *)ctx->plan.p;
-    } else {
-        grid = grid_for(n_rows);
-    }
-    // rows whose walk meets a sign the FP64 filter cannot certify are queued
-    // for the exact kernel, with their MC words
-    const int64_t wpr = is_mc ? 3 * (int64_t)a.M : 0;
-    int64_t cap = std::min<int64_t>(n_rows, std::max<int64_t>(1 << 16, n_rows / 16));
-    if ((rc = ensure(ctx, ctx->defer_rows, cap * sizeof(int64_t)))) return rc;
-    if (wpr && (rc = ensure(ctx, ctx->defer_words, cap * wpr * sizeof(uint32_t)))) return rc;
-    a.defer_rows = (int64_t *)ctx->defer_rows.p;
-    a.defer_words = (uint32_t *)ctx->defer_words.p;
-    a.defer_cap = (int)cap;
-    const bool f64 = vf->dtype == DTFE_F64;
-    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    launch_any(a, f64, want_vel, ns, 0, is_mc, grid, lds, 0, ctx->stream);
-    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-    HIP_TRY(hipGetLastError());
-    int st[4];
-    HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    ctx->last_ms[0] = ms;
-    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-    ctx->last_ms[2] = ms;
-    ctx->last_ms[1] = 0.0;
-    ctx->last_deferred = st[1];
-    if (st[2]) {
-        // queue overflow: redo everything with exact signs
-        launch_any(a, f64, want_vel, ns, 2, is_mc, grid, lds, 0, ctx->stream);
-    } else if (st[1] > 0) {
-        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-        launch_any(a, f64, want_vel, ns, 1, is_mc, grid_for(st[1]), 0, st[1], ctx->stream);
-        HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
-        ctx->ev_pending[1] = true;
-    }
-    HIP_TRY(hipGetLastError());
-    if (st[1] > 0 || st[2]) {
-        HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-    }
-    if (st[0] > 0)
-        return fail(ctx, DTFE_ERR_WALK, std::to_string(st[0]) + " point locations did not terminate: the mesh is not a "
-                                                              "valid triangulation");
-    return DTFE_OK;
+    return launch_rows(ctx, a, mc, vf->dtype == DTFE_F64, want_vel, ns, false);
 }
 
 int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, int dtype, const void *weight,
@@ -1585,6 +1966,113 @@ int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, i
     return DTFE_OK;
 }
 
+int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *vf, const void *weight,
+                     double rho_bar, const dtfe_sampling *smp, const dtfe_mc *mc, unsigned flags,
+                     const dtfe_upstream *up, dtfe_vertex_grads *gr) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    int rc = check_mesh(ctx, m);
+    if (rc) return rc;
+    if (!vf || !smp || !mc || !up || !gr) return fail(ctx, DTFE_ERR_ARGUMENT, "NULL argument");
+    if (vf->dtype != DTFE_F32 && vf->dtype != DTFE_F64) return fail(ctx, DTFE_ERR_ARGUMENT, "bad dtype");
+    if (flags & DTFE_AVERAGE_FINITE_ONLY)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "the adjoint supports the reference's division by M only");
+    const int method = mc->method;
+    if (method != DTFE_METHOD_CENTRE && method != DTFE_METHOD_MC && method != DTFE_METHOD_EQUI)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "method must be 0, 2 or 3");
+    if (method != DTFE_METHOD_CENTRE && mc->samples < 1) return fail(ctx, DTFE_ERR_ARGUMENT, "samples must be >= 1");
+    const bool vel_up = up->velocity || up->velocity_gradient || up->velocity_divergence || up->velocity_shear ||
+                        up->velocity_vorticity;
+    if (!vf->density) return fail(ctx, DTFE_ERR_ARGUMENT, "the adjoint needs the forward vertex densities");
+    if (vel_up && !vf->velocity) return fail(ctx, DTFE_ERR_ARGUMENT, "velocity upstream gradients without velocities");
+    if (gr->weight && !weight) return fail(ctx, DTFE_ERR_ARGUMENT, "weight gradients need the weights");
+    if (!(rho_bar > 0.0)) return fail(ctx, DTFE_ERR_ARGUMENT, "average density must be positive");
+    HIP_TRY(hipSetDevice(ctx->device));
+    gr->rho_bar = 0.0;
+    const int64_t nv = m->n_vertices, nc = m->n_cells;
+    const bool f64 = vf->dtype == DTFE_F64;
+    const size_t esz = f64 ? 8 : 4;
+    auto zero = [&](void *p, int64_t n) {
+        if (!p || n <= 0) return;
+        if (f64) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (double *)p, n, 0.0);
+        else hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (float *)p, n, 0.0f);
+    };
+    zero(gr->position, 3 * nv);
+    zero(gr->weight, nv);
+    zero(gr->velocity, 3 * nv);
+    if (nv < 4 || nc == 0) return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: gradients are zero");
+    if ((rc = ensure(ctx, ctx->adj_rho, nv * esz))) return rc;
+    zero(ctx->adj_rho.p, nv);
+
+    SampleArgs a{};
+    a.pos = m->pos;
+    a.cells = (const int4 *)m->cells;
+    a.nbrs = (const int4 *)m->neighbors;
+    a.nc = nc;
+    a.vf = VertexFieldsDev{vf->density, vf->velocity, nullptr, 0};
+    a.flags = flags;
+    a.kind = smp->kind;
+    a.method = method;
+    a.M = method == DTFE_METHOD_CENTRE ? 1 : mc->samples;
+    a.arg_order = mc->arg_order;
+    a.g_density = up->density;
+    a.g_velocity = up->velocity;
+    a.g_grad = up->velocity_gradient;
+    a.g_div = up->velocity_divergence;
+    a.g_shear = up->velocity_shear;
+    a.g_vort = up->velocity_vorticity;
+    a.d_rho = ctx->adj_rho.p;
+    a.d_vel = gr->velocity;
+    a.d_pos = gr->position;
+    std::vector<double> axes, trig;
+    if ((rc = setup_rows(ctx, smp, method, a, axes, trig))) return rc;
+    if (a.n_rows > 0) {
+        if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
+        if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true))) return rc;
+    }
+    // vertex density chain
+    if (up->density && (gr->weight || gr->position)) {
+        if ((rc = ensure(ctx, ctx->W, nv * sizeof(double)))) return rc;
+        if ((rc = ensure(ctx, ctx->hull, nv))) return rc;
+        if ((rc = ensure(ctx, ctx->adj_dW, nv * sizeof(double)))) return rc;
+        const int parts = std::min(1024, grid_for(nv));
+        if ((rc = ensure(ctx, ctx->adj_partial, parts * sizeof(double)))) return rc;
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->W.p, nv,
+                           0.0);
+        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (uint8_t *)ctx->hull.p,
+                           nv, (uint8_t)0);
+        hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+                           (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
+                           (uint8_t *)ctx->hull.p);
+        if (f64)
+            hipLaunchKernelGGL(k_adj_vertex<double>, dim3(parts), dim3(kBlock), 0, ctx->stream,
+                               (const double *)weight, (const double *)vf->density, (const double *)ctx->W.p,
+                               (const uint8_t *)ctx->hull.p, (const double *)ctx->adj_rho.p, nv, rho_bar,
+                               (double *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+        else
+            hipLaunchKernelGGL(k_adj_vertex<float>, dim3(parts), dim3(kBlock), 0, ctx->stream,
+                               (const float *)weight, (const float *)vf->density, (const double *)ctx->W.p,
+                               (const uint8_t *)ctx->hull.p, (const float *)ctx->adj_rho.p, nv, rho_bar,
+                               (float *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+        if (gr->position) {
+            if (f64)
+                hipLaunchKernelGGL(k_adj_tets<double>, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+                                   (const int4 *)m->cells, nc, (const double *)ctx->adj_dW.p, (double *)gr->position);
+            else
+                hipLaunchKernelGGL(k_adj_tets<float>, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+                                   (const int4 *)m->cells, nc, (const double *)ctx->adj_dW.p, (float *)gr->position);
+        }
+        std::vector<double> part(parts);
+        HIP_TRY(hipMemcpyAsync(part.data(), ctx->adj_partial.p, parts * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        double sum = 0.0;
+        for (double x : part) sum += x;
+        gr->rho_bar = sum;
+    }
+    HIP_TRY(hipGetLastError());
+    return DTFE_OK;
+}
+
 // Rows of the last sampling call that needed exact signs.
 int dtfe_hip_last_deferred(dtfe_ctx *ctx) { return ctx ? ctx->last_deferred : -1; }
 
@@ -1606,6 +2094,15 @@ int dtfe_hip_last_timings(dtfe_ctx *ctx, double *ms) {
         ctx->ev_pending[3] = false;
     }
     for (int i = 0; i < 4; ++i) ms[i] = ctx->last_ms[i];
+    return DTFE_OK;
+}
+
+// Walk statistics of the last sampling call: tetrahedra visited by the
+// centre walks and by the sample walks, rows, samples. The step counts are
+// only gathered by builds with DTFE_COUNT_STEPS (zero otherwise).
+int dtfe_hip_last_walk_stats(dtfe_ctx *ctx, int64_t *out) {
+    if (!ctx || !out) return DTFE_ERR_ARGUMENT;
+    for (int i = 0; i < 4; ++i) out[i] = ctx->last_walk[i];
     return DTFE_OK;
 }
 

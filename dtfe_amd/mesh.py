@@ -122,8 +122,12 @@ def _qhull(pts: np.ndarray):
     return cells, nbrs
 
 
-def tessellate(points: np.ndarray, backend: str = "auto") -> Mesh:
-    """Delaunay tessellation of the points as a flattened mesh."""
+def tessellate(points: np.ndarray, backend: str = "auto", spatial_order: bool = True) -> Mesh:
+    """Delaunay tessellation of the points as a flattened mesh.
+
+    With spatial_order (the default) vertices and tetrahedra are renumbered
+    along a Morton curve, so that neighbours in space are neighbours in
+    memory; point_of_vertex / vertex_of_point keep the link to the input."""
     t0 = time.perf_counter()
     points = np.ascontiguousarray(points, dtype=np.float64)
     kept, vop = _dedupe_keep_last(points)
@@ -152,6 +156,13 @@ def tessellate(points: np.ndarray, backend: str = "auto") -> Mesh:
         if np.any(neg):
             cells[neg] = cells[neg][:, [0, 1, 3, 2]]
             nbrs[neg] = nbrs[neg][:, [0, 1, 3, 2]]
+    if spatial_order and cells.shape[0]:
+        from . import native_delaunay
+        pos, cells, nbrs, perm = native_delaunay.spatial_order(pos, cells, nbrs)
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size)
+        kept = kept[perm]
+        vop = inv[vop]
     m = Mesh(pos=pos, cells=np.ascontiguousarray(cells, dtype=np.int32),
              nbrs=np.ascontiguousarray(nbrs, dtype=np.int32), point_of_vertex=kept,
              vertex_of_point=vop, backend=backend, seconds=time.perf_counter() - t0)

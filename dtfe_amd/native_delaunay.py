@@ -27,6 +27,8 @@ def _load():
         lib.dtfe_tess_copy.argtypes = [vp, vp, vp, vp]
         lib.dtfe_tess_free.argtypes = [vp]
         lib.dtfe_orient_sign.argtypes = [vp, vp, vp, vp]
+        lib.dtfe_mesh_spatial_order.argtypes = [C.c_int64, vp, C.c_int64, vp, vp, vp]
+        lib.dtfe_mesh_spatial_order.restype = C.c_int
         lib.dtfe_lifted_sign.argtypes = [vp, vp, vp, vp, vp]
         _lib = lib
     return _lib
@@ -55,6 +57,22 @@ def delaunay(points: np.ndarray):
         return cells, nbrs
     finally:
         lib.dtfe_tess_free(h)
+
+
+def spatial_order(pos: np.ndarray, cells: np.ndarray, nbrs: np.ndarray):
+    """Morton renumbering of vertices and tetrahedra (in place on copies).
+    Returns (pos, cells, nbrs, vertex_perm) with vertex_perm[new] = old."""
+    lib = _load()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    cells = np.ascontiguousarray(cells, dtype=np.int32).copy()
+    nbrs = np.ascontiguousarray(nbrs, dtype=np.int32).copy()
+    perm = np.empty(pos.shape[0], dtype=np.int64)
+    rc = lib.dtfe_mesh_spatial_order(pos.shape[0], pos.ctypes.data_as(C.c_void_p), cells.shape[0],
+                                     cells.ctypes.data_as(C.c_void_p), nbrs.ctypes.data_as(C.c_void_p),
+                                     perm.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise ValueError("bad mesh arrays")
+    return pos, cells, nbrs, perm
 
 
 def orient_sign(a, b, c, d) -> int:

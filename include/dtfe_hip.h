@@ -156,6 +156,35 @@ int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, i
                      const void *weight, const void *velocity, const double box[6],
                      const int64_t n[3], double rho_bar, unsigned fields, dtfe_outputs *out);
 
+/* ---- adjoint (no counterpart in the reference) ------------------------------------------
+ * Reverse mode of dtfe_hip_vertex_density + dtfe_hip_sample at fixed
+ * tessellation: given dL/d(output rows), returns dL/d(vertex positions,
+ * weights, velocities). Sample points are re-located and their barycentric
+ * coordinates recomputed (nothing is stored by the forward). Outputs are
+ * overwritten; NULL outputs are skipped. rho_bar is held fixed inside the
+ * chain and dL/d rho_bar is returned, so a caller whose rho_bar is
+ * sum(w)/35000^3 adds rho_bar_grad / 35000^3 to every particle weight. */
+typedef struct {
+    const void *density;             /* [n]    dL/d density, or NULL */
+    const void *velocity;            /* [n][3] */
+    const void *velocity_gradient;   /* [n][9] */
+    const void *velocity_divergence; /* [n]    */
+    const void *velocity_shear;      /* [n][5] */
+    const void *velocity_vorticity;  /* [n][3] */
+} dtfe_upstream;
+
+typedef struct {
+    void *position;   /* [nv][3] dL/dx (field dtype) or NULL */
+    void *weight;     /* [nv]    dL/dw or NULL */
+    void *velocity;   /* [nv][3] dL/dv or NULL */
+    double rho_bar;   /* set on return: dL/d rho_bar */
+} dtfe_vertex_grads;
+
+/* vf->density must hold the forward vertex densities; weight the vertex weights. */
+int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *mesh, const dtfe_vertex_fields *vf, const void *weight,
+                     double rho_bar, const dtfe_sampling *sampling, const dtfe_mc *mc, unsigned flags,
+                     const dtfe_upstream *upstream, dtfe_vertex_grads *grads);
+
 /* Library version string. */
 const char *dtfe_hip_version(void);
 
@@ -168,6 +197,11 @@ int dtfe_hip_last_timings(dtfe_ctx *ctx, double ms[4]);
 int dtfe_hip_last_deferred(dtfe_ctx *ctx);
 /* Point locations of the last dtfe_hip_sample call that did not terminate. */
 int dtfe_hip_walk_failures(dtfe_ctx *ctx, int *failures);
+/* Walk statistics of the last dtfe_hip_sample call: out[0] tetrahedra
+ * visited by the cell-centre walks, out[1] by the sample walks (both only
+ * counted by builds with -DDTFE_COUNT_STEPS, else 0), out[2] rows,
+ * out[3] samples. */
+int dtfe_hip_last_walk_stats(dtfe_ctx *ctx, int64_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,36 @@
+"""FP64 pure-tensor restatement of the differentiable DTFE forward (test
+infrastructure): vertex densities from tetrahedron volumes, barycentric
+interpolation at fixed located tetrahedra, velocity gradient from the
+barycentric gradients. torch.autograd through it gives the reference
+gradients the hand-written adjoint is checked against."""
+from __future__ import annotations
+
+import torch
+
+
+def forward(pos, w, v, cells, hull, located, pts, rho_bar, value_bug=True):
+    """pos [nv,3], w [nv], v [nv,3] float64 tensors (may require grad);
+    cells [nc,4] long, hull [nv] bool, located [n] long (>= 0), pts [n,3].
+    Returns density, velocity, velocity_gradient ([n,9], grad[a*3+b] = dv_a/dx_b)."""
+    nv = pos.shape[0]
+    X = pos[cells]
+    e = X[:, 1:] - X[:, :1]
+    V = torch.linalg.det(e) / 6.0
+    W = torch.zeros(nv, dtype=pos.dtype).index_add(0, cells.reshape(-1), V.repeat_interleave(4))
+    rho = torch.where(hull, torch.zeros_like(w), 4.0 * w / (rho_bar * W))
+    c = cells[located]
+    Xt = pos[c]
+    A = Xt[:, 1:] - Xt[:, :1]              # rows x_{k+1} - x_0
+    Ai = torch.linalg.inv(A)               # Ai[b][k]: d lambda_{k+1} / d x_b
+    rel = pts - Xt[:, 0]
+    lam123 = torch.einsum("nb,nbk->nk", rel, Ai)
+    lam = torch.cat([1.0 - lam123.sum(1, keepdim=True), lam123], 1)
+    gl = torch.cat([-Ai.sum(2, keepdim=True), Ai], 2).transpose(1, 2)   # [n, 4(i), 3(b)]
+    dens = (lam * rho[c]).sum(1)
+    vv = v[c]                               # [n, 4, 3(a)]
+    G = torch.einsum("nia,nib->nab", vv, gl)
+    if value_bug:
+        vel = vv[:, 0] + G[:, :, 2] * (pts[:, 2] - Xt[:, 0, 2])[:, None]
+    else:
+        vel = torch.einsum("ni,nia->na", lam, vv)
+    return dens, vel, G.reshape(-1, 9)

@@ -1,0 +1,150 @@
+"""The hand-written adjoint against (1) torch autograd through an FP64
+pure-tensor forward at the same located tetrahedra and (2) central finite
+differences of the GPU forward at fixed tessellation."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from dtfe_amd import mesh as M
+from oracle import oracle as O
+from tests import adjoint_reference as R
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(prob, dtype):
+    from dtfe_amd import api
+    dm = api.DeviceMesh.from_mesh(prob.mesh, "cuda:0")
+    w = torch.from_numpy(prob.w).to("cuda:0", dtype)
+    v = torch.from_numpy(prob.vel).to("cuda:0", dtype)
+    rho = api.vertex_density(dm, w, prob.rho_bar)
+    return dm, w, v, rho
+
+
+def _grid(prob, n):
+    box = prob.region
+    xs = [box[2 * a] + (np.arange(n[a]) + 0.5) * (box[2 * a + 1] - box[2 * a]) / n[a] for a in range(3)]
+    X, Y, Z = np.meshgrid(*xs, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+
+
+@pytest.mark.parametrize("value_bug", [True, False])
+def test_adjoint_matches_fp64_autograd(gpu, value_bug):
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    m = prob.mesh
+    n = (12, 12, 12)
+    spec = api.GridSpec(prob.region, n)
+    dm, w, v, rho = _setup(prob, torch.float64)
+    fields = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity"]
+    out = api.sample(dm, rho, v, None, fields, spec, method=0, value_bug=value_bug, want_cells=True)
+    located = out["located_cell"].cpu().numpy().astype(np.int64)
+    assert (located >= 0).all()
+    rng = np.random.default_rng(3)
+    up = {f: torch.from_numpy(rng.standard_normal(tuple(out[f].shape))).cuda() for f in fields}
+    g, g_rb = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=0, value_bug=value_bug)
+
+    pos = torch.from_numpy(m.pos).clone().requires_grad_(True)
+    wt = torch.from_numpy(prob.w).clone().requires_grad_(True)
+    vt = torch.from_numpy(prob.vel).clone().requires_grad_(True)
+    _, _, hull = O.vertex_density(m, prob.w, prob.rho_bar)
+    rb = torch.tensor(prob.rho_bar, dtype=torch.float64, requires_grad=True)
+    d, vel, G = R.forward(pos, wt, vt, torch.from_numpy(m.cells).long(), torch.from_numpy(hull),
+                          torch.from_numpy(located), torch.from_numpy(_grid(prob, n)), rb, value_bug)
+    div = G[:, 0] + G[:, 4] + G[:, 8]
+    vort = torch.stack([(G[:, 1] - G[:, 3]) / 2, (G[:, 2] - G[:, 6]) / 2, (G[:, 5] - G[:, 7]) / 2], 1)
+    # forward agreement
+    assert torch.allclose(d, out["density"].cpu(), rtol=1e-10, atol=0)
+    L = ((d * up["density"].cpu()).sum() + (vel * up["velocity"].cpu()).sum() +
+         (G * up["velocity_gradient"].cpu()).sum() + (div * up["velocity_divergence"].cpu()).sum() +
+         (vort * up["velocity_vorticity"].cpu()).sum())
+    L.backward()
+    for name, ref in (("position", pos.grad), ("weight", wt.grad), ("velocity", vt.grad)):
+        got = g[name].cpu()
+        err = (got - ref).abs().max().item() / ref.abs().max().item()
+        print(f"adjoint bug={value_bug} {name}: max rel err {err:.3e}")
+        assert err < 1e-9, name
+    err_rb = abs(g_rb - rb.grad.item()) / abs(rb.grad.item())
+    print(f"adjoint d rho_bar rel err {err_rb:.3e}")
+    assert err_rb < 1e-9
+
+
+def test_adjoint_mixed_precision(gpu):
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    n = (12, 12, 12)
+    spec = api.GridSpec(prob.region, n)
+    dm64, w64, v64, rho64 = _setup(prob, torch.float64)
+    dm32, w32, v32, rho32 = _setup(prob, torch.float32)
+    fields = ["density", "velocity_divergence"]
+    out = api.sample(dm64, rho64, v64, None, fields, spec, method=0)
+    rng = np.random.default_rng(4)
+    up = {f: torch.from_numpy(rng.standard_normal(tuple(out[f].shape))).cuda() for f in fields}
+    g64, _ = api.adjoint(dm64, rho64, w64, v64, prob.rho_bar, spec, up, method=0)
+    g32, _ = api.adjoint(dm32, rho32, w32, v32, prob.rho_bar, spec, {k: x.float() for k, x in up.items()}, method=0)
+    for name in ("position", "weight", "velocity"):
+        a, b = g32[name].double().cpu(), g64[name].cpu()
+        err = (a - b).abs().max().item() / b.abs().max().item()
+        print(f"mixed adjoint {name}: {err:.3e}")
+        assert err < 1e-4, name
+
+
+def test_adjoint_central_differences(gpu):
+    """Central differences of the GPU FP64 forward at fixed tessellation,
+    for coordinates whose +-h perturbation keeps every located tetrahedron."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    m = prob.mesh
+    n = (10, 10, 10)
+    spec = api.GridSpec(prob.region, n)
+    dm, w, v, rho = _setup(prob, torch.float64)
+    fields = ["density", "velocity_divergence"]
+    out = api.sample(dm, rho, v, None, fields, spec, method=0, want_cells=True)
+    rng = np.random.default_rng(5)
+    wc = torch.from_numpy(rng.uniform(0.5, 1.5, out["density"].shape[0])).cuda()
+    tgt_d = out["density"] * 1.1
+    tgt_t = out["velocity_divergence"] + 1.0
+
+    def loss_of(pos_np, w_np, v_np):
+        dmp = api.DeviceMesh(pos=torch.from_numpy(pos_np).cuda(), cells=dm.cells, nbrs=dm.nbrs)
+        wt = torch.from_numpy(w_np).cuda()
+        r = api.vertex_density(dmp, wt, prob.rho_bar)
+        o = api.sample(dmp, r, torch.from_numpy(v_np).cuda(), None, fields, spec, method=0, want_cells=True)
+        L = (wc * (o["density"] - tgt_d) ** 2).sum() + ((o["velocity_divergence"] - tgt_t) ** 2).sum()
+        return L.item(), o["located_cell"]
+
+    L0, cells0 = loss_of(m.pos, prob.w, prob.vel)
+    up = {"density": 2 * wc * (out["density"] - tgt_d), "velocity_divergence": 2 * (out["velocity_divergence"] - tgt_t)}
+    g, _ = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=0)
+    gp, gw, gv = g["position"].cpu().numpy(), g["weight"].cpu().numpy(), g["velocity"].cpu().numpy()
+    # vertices touched by the located tetrahedra
+    touched = np.unique(m.cells[cells0.cpu().numpy()].ravel())
+    checked = 0
+    for vtx in rng.choice(touched, 12, replace=False):
+        for kind in ("pos", "w", "v"):
+            comp = int(rng.integers(0, 3))
+            base = {"pos": m.pos, "w": prob.w, "v": prob.vel}[kind]
+            h = 1e-6 * (20.0 if kind == "pos" else max(1.0, abs(float(base.reshape(len(base), -1)[vtx, 0]))))
+            vals = []
+            ok = True
+            for sgn in (1, -1):
+                arrs = {"pos": m.pos.copy(), "w": prob.w.copy(), "v": prob.vel.copy()}
+                if kind == "w":
+                    arrs["w"][vtx] += sgn * h
+                else:
+                    arrs[kind][vtx, comp] += sgn * h
+                L, c = loss_of(arrs["pos"], arrs["w"], arrs["v"])
+                ok = ok and torch.equal(c, cells0)
+                vals.append(L)
+            if not ok:
+                continue
+            fd = (vals[0] - vals[1]) / (2 * h)
+            an = {"pos": gp[vtx, comp], "w": gw[vtx], "v": gv[vtx, comp]}[kind]
+            scale = max(abs(fd), abs(an), 1e-8 * abs(L0))
+            assert abs(fd - an) <= 1e-4 * scale + 1e-7 * abs(L0), (kind, vtx, comp, fd, an)
+            checked += 1
+    print(f"central differences checked: {checked}")
+    assert checked >= 20

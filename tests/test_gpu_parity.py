@@ -234,3 +234,38 @@ def test_deposits(gpu, scheme, dtype, tol):
     e2 = rel_err(out["velocity"].cpu().numpy(), v)
     print(f"{scheme} {dtype}: density {e1:.3e} velocity {e2:.3e}")
     assert e1 < tol and e2 < (tol * 100 if dtype == torch.float32 else tol)
+
+
+def test_lattice_ties(gpu):
+    """Cospherical lattice (native exact tessellation): cell-centre samples on
+    faces, edges and vertices; exact signs and the symbolic tie-break."""
+    from dtfe_amd import api
+    prob = P.lattice_problem()
+    dm, rho, vel, _ = gpu_setup(prob, torch.float64)
+    n = (12, 12, 12)
+    out = api.sample(dm, rho, vel, None, ["density", "velocity_gradient"], api.GridSpec(prob.region, n),
+                     method=0, want_cells=True)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=("density", "velocity_gradient"),
+                   kind="grid", box=prob.region, n=n, method=0)
+    assert np.array_equal(out["located_cell"].cpu().numpy(), ref["located_cell"])
+    for f in ("density", "velocity_gradient"):
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"lattice {f}: {e:.3e}")
+        assert e < TOL64
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_derived_fields_without_gradient(gpu, method):
+    """Divergence, shear and vorticity requested without the gradient itself."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, _ = gpu_setup(prob, torch.float64)
+    n = (10, 12, 14)
+    fields = ["velocity_divergence", "velocity_shear", "velocity_vorticity"]
+    out = api.sample(dm, rho, vel, None, fields, api.GridSpec(prob.region, n), method=method, samples=4, seed=3)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=("velocity_gradient",) + tuple(fields),
+                   kind="grid", box=prob.region, n=n, method=method, samples=4, seed=3)
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"derived only method={method} {f}: {e:.3e}")
+        assert e < TOL64, f
