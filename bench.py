@@ -41,10 +41,13 @@ def parse():
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--tess", default="auto")
     ap.add_argument("--mesh-cache", default="")
-    ap.add_argument("--cpu-slabs", type=int, default=2, help="x-slabs of the grid timed on the CPU oracle")
+    ap.add_argument("--cpu-slabs", type=int, default=256,
+                    help="x-slabs of the grid timed on the CPU oracle (256 = the whole 256^3 grid, ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of this many MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -75,6 +78,38 @@ def build_mesh(args):
 FIELDS = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity"]
 OUT_FLOATS = 1 + 3 + 9 + 1 + 3
 
+# HBM bytes per launch of the per-sample kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE on the default workload, each scaled by the counter's reading on a
+# plain copy of known size (the summary states the commands and the factors)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_k_sample_mc_lanes_clustered128_grid256_mc8.md")
+
+
+def profiled_traffic(kernel):
+    """Measured HBM bytes per launch of `kernel` from the committed summary,
+    or None when it holds no figure for it."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    key = f"hbm_bytes_per_launch[{kernel}] = "
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            line = line.strip().strip("`")
+            if line.startswith(key):
+                return int(line[len(key):].split()[0])
+    return None
+
+
+def copy_probe(mib, steps):
+    """A plain device-to-device copy of `mib` MiB, `steps` times: reads and
+    writes exactly mib MiB per launch, the yardstick for the byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_per_launch": n * 4, "launches": steps}), flush=True)
+
 
 def main():
     args = parse()
@@ -82,6 +117,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe, max(1, args.steps))
+        return
     import torch
     from dtfe_amd import api, mesh as M
 
@@ -122,15 +160,20 @@ def main():
     k_exact_ms = float(kern[:, 1].mean())
     k_seed_ms = float(kern[:, 2].mean())
     k_vd_ms = float(kern[:, 3].mean())
+    k_centre_ms = float(kern[:, 4].mean())
+    lanes = args.samples in (2, 4, 8, 16)
+    kernel = "k_sample_mc_lanes" if lanes else "k_sample_mc"
     n_cells = int(np.prod(n))
     samples = n_cells * args.samples
     value = samples / (ms_step * 1e-3)
 
-    # algorithmic bytes of the sampling kernel: tetrahedron ids, vertex
+    # algorithmic bytes of the per-sample kernel: tetrahedron ids, vertex
     # positions / densities / velocities read once, outputs written once
     esz = 4 if dt == torch.float32 else 8
     B_alg = 16 * m.n_cells + m.n_vertices * (24 + esz + 3 * esz) + esz * OUT_FLOATS * n_cells
     achieved = B_alg / (k_sample_ms * 1e-3) / 1e9
+    default_workload = (args.n_side, args.grid, args.samples, args.dtype, args.seed) == (128, 256, 8, "f32", 2)
+    traffic = profiled_traffic(kernel) if default_workload else None
 
     # all-FP64 comparison path on the same mesh
     fp64 = None
@@ -181,14 +224,15 @@ def main():
             "field_dtype": args.dtype, "tessellation": m.backend,
             "tessellation_s": round(m.seconds, 3),
         },
-        "kernels_ms": {"sample": k_sample_ms, "exact": k_exact_ms, "seed_grid": k_seed_ms,
-                       "vertex_density": k_vd_ms},
+        "kernels_ms": {"sample": k_sample_ms, "locate_centres": k_centre_ms, "exact": k_exact_ms,
+                       "seed_grid": k_seed_ms, "vertex_density": k_vd_ms},
         "deferred_rows": deferred,
         "walk_steps": {"centre_per_row": walk[0] / max(1, walk[2]), "per_sample": walk[1] / max(1, walk[3])}
         if walk[0] else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_sample_mc", "algorithmic_bytes": B_alg},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "kernel": kernel, "algorithmic_bytes": B_alg,
+                     "kernel_ms": k_sample_ms},
         "fp64_path": fp64,
         "fwd_adjoint_ratio": None,
         "cpu_baseline": cpu,

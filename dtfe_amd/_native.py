@@ -49,13 +49,15 @@ EXPORTED = [
     "dtfe_hip_create", "dtfe_hip_destroy", "dtfe_hip_set_stream", "dtfe_hip_synchronize",
     "dtfe_hip_last_error", "dtfe_hip_vertex_density", "dtfe_hip_sample", "dtfe_hip_deposit",
     "dtfe_hip_version", "dtfe_hip_last_timings", "dtfe_hip_last_deferred", "dtfe_hip_walk_failures",
-    "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint",
+    "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint", "dtfe_hip_cell_records",
 ]
+
+CELL_RECORD_BYTES = 128
 
 
 class Mesh(C.Structure):
     _fields_ = [("n_vertices", C.c_int64), ("n_cells", C.c_int64), ("pos", C.c_void_p),
-                ("cells", C.c_void_p), ("neighbors", C.c_void_p)]
+                ("cells", C.c_void_p), ("neighbors", C.c_void_p), ("cell_records", C.c_void_p)]
 
 
 class VertexFields(C.Structure):
@@ -122,13 +124,14 @@ def load(path: str = LIB_PATH):
         lib.dtfe_hip_last_error.restype = C.c_char_p
         lib.dtfe_hip_version.restype = C.c_char_p
         lib.dtfe_hip_vertex_density.argtypes = [vp, C.POINTER(Mesh), i32, vp, C.c_double, vp]
+        lib.dtfe_hip_cell_records.argtypes = [vp, C.POINTER(Mesh), vp]
         lib.dtfe_hip_sample.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), C.c_uint,
                                         C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Outputs)]
         lib.dtfe_hip_deposit.argtypes = [vp, i32, i64, vp, i32, vp, vp, C.POINTER(C.c_double * 6),
                                          C.POINTER(C.c_int64 * 3), C.c_double, C.c_uint, C.POINTER(Outputs)]
         lib.dtfe_hip_last_deferred.argtypes = [vp]
         lib.dtfe_hip_walk_failures.argtypes = [vp, C.POINTER(C.c_int)]
-        lib.dtfe_hip_last_timings.argtypes = [vp, C.POINTER(C.c_double * 4)]
+        lib.dtfe_hip_last_timings.argtypes = [vp, C.POINTER(C.c_double * 5)]
         lib.dtfe_hip_last_walk_stats.argtypes = [vp, C.POINTER(C.c_int64 * 4)]
         lib.dtfe_hip_adjoint.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), vp, C.c_double,
                                          C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Upstream),
@@ -164,8 +167,9 @@ class Context:
         self.check(self.lib.dtfe_hip_synchronize(self.h))
 
     def last_timings(self):
-        """(sample kernel, exact kernel, seed grid, vertex density) in ms."""
-        t = (C.c_double * 4)()
+        """(sample kernel, exact kernel, seed grid, vertex density, centre
+        location) in ms."""
+        t = (C.c_double * 5)()
         self.check(self.lib.dtfe_hip_last_timings(self.h, C.byref(t)))
         return tuple(t)
 

@@ -27,6 +27,8 @@
 #include <vector>
 
 #include "../../include/dtfe_hip.h"
+#define DTFE_HD __host__ __device__
+#include "fastdiv.h"
 #include "mt19937.h"
 #include "predicates.h"
 
@@ -44,6 +46,10 @@ constexpr int kBlock = 256;
 #ifndef DTFE_SEED_BUCKETS_PER_VERTEX
 #define DTFE_SEED_BUCKETS_PER_VERTEX 0.5
 #endif
+// occupancy floor of the one-thread-per-sample Monte-Carlo kernel
+#ifndef DTFE_LANES_MIN_WAVES
+#define DTFE_LANES_MIN_WAVES 4
+#endif
 #if DTFE_MIN_WAVES > 0
 #define DTFE_SAMPLE_BOUNDS __launch_bounds__(kBlock, DTFE_MIN_WAVES)
 #else
@@ -51,6 +57,24 @@ constexpr int kBlock = 256;
 #endif
 constexpr int kMaxScalars = 4;
 constexpr int kMaxWalkSteps = 1 << 20;
+// Workgroup size of the one-thread-per-sample Monte-Carlo kernel.
+#ifndef DTFE_LANES_BLOCK
+#define DTFE_LANES_BLOCK 64
+#endif
+constexpr int kLanesBlock = DTFE_LANES_BLOCK;
+// Its LDS ring: a round of kLanesBlock samples reads 3 kLanesBlock consecutive
+// words, which touch at most (3 kLanesBlock + 622) / 624 + 1 MT blocks, plus
+// the block before them that generating the next one reads.
+constexpr int kRingLanes = (3 * kLanesBlock + 622) / 624 + 2;
+
+// One tetrahedron as the point-location walk reads it: 128 bytes, one L2 line,
+// so a walk step is a single dependent memory round trip.
+struct __align__(16) CellRecord {
+    double x[12];  // vertex positions x0 y0 z0 x1 ... z3
+    int4 v;        // vertex ids
+    int4 nb;       // neighbour across the face opposite vertex i, -1 on the hull
+};
+static_assert(sizeof(CellRecord) == DTFE_CELL_RECORD_BYTES, "cell record layout");
 
 // ----------------------------------------------------------------------------
 // small helpers
@@ -126,32 +150,34 @@ __device__ __forceinline__ void face_orients(const d3 X[4], d3 p, double O[4], i
     O[1] = -dot(d0, x23);
     O[2] = -dot(d0, x31);
     O[3] = -dot(d0, x12);
-    d3 p23 = abs_cross(d2, d3v), p31 = abs_cross(d3v, d1), p12 = abs_cross(d1, d2);
-    d3 a0 = dabs(d0), a1 = dabs(d1);
-    const double eb = 8.0e-16;  // (7 + 56 eps) eps, rounded up
-    double b0 = eb * dot(a1, p23), b1 = eb * dot(a0, p23), b2 = eb * dot(a0, p31), b3 = eb * dot(a0, p12);
-    cert[0] = O[0] > b0 ? 1 : (O[0] < -b0 ? -1 : 0);
-    cert[1] = O[1] > b1 ? 1 : (O[1] < -b1 ? -1 : 0);
-    cert[2] = O[2] > b2 ? 1 : (O[2] < -b2 ? -1 : 0);
-    cert[3] = O[3] > b3 ? 1 : (O[3] < -b3 ? -1 : 0);
+    // Shewchuk's orient3d bound (7 + 56 eps) eps * permanent, with every
+    // permanent (six products of three |differences|) bounded by 6 mx^3,
+    // mx = the largest |difference|: one bound for all four faces.
+    double mx = fmax(fmax(fmax(fabs(d0.x), fabs(d0.y)), fmax(fabs(d0.z), fabs(d1.x))),
+                     fmax(fmax(fabs(d1.y), fabs(d1.z)), fmax(fabs(d2.x), fabs(d2.y))));
+    mx = fmax(mx, fmax(fmax(fabs(d2.z), fabs(d3v.x)), fmax(fabs(d3v.y), fabs(d3v.z))));
+    const double eb = 6.0 * 8.0e-16;  // 6 x (7 + 56 eps) eps, rounded up
+    double b = eb * (mx * mx * mx);
+    cert[0] = O[0] > b ? 1 : (O[0] < -b ? -1 : 0);
+    cert[1] = O[1] > b ? 1 : (O[1] < -b ? -1 : 0);
+    cert[2] = O[2] > b ? 1 : (O[2] < -b ? -1 : 0);
+    cert[3] = O[3] > b ? 1 : (O[3] < -b ? -1 : 0);
 }
 
 // EXACT = false: signs the FP64 filter cannot certify end the walk with
 // t = -3 (the row is then redone by the exact kernel). EXACT = true: such
 // signs are decided exactly with the symbolic tie-break.
 template <bool EXACT>
-__device__ Located locate(const double *__restrict__ pos, const int4 *__restrict__ cells,
-                          const int4 *__restrict__ nbrs, d3 p, int t) {
+__device__ Located locate(const CellRecord *__restrict__ rec, d3 p, int t) {
     Located L;
     int prev = -1;
     for (int step = 0; step < kMaxWalkSteps; ++step) {
         L.steps = step + 1;
-        int4 c = cells[t];
-        int4 nb4 = nbrs[t];
-        L.X[0] = load_pos(pos, c.x);
-        L.X[1] = load_pos(pos, c.y);
-        L.X[2] = load_pos(pos, c.z);
-        L.X[3] = load_pos(pos, c.w);
+        const CellRecord r = rec[t];
+        const int4 c = r.v;
+        const int4 nb4 = r.nb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) L.X[k] = d3{r.x[3 * k], r.x[3 * k + 1], r.x[3 * k + 2]};
         int cert[4];
         face_orients(L.X, p, L.O, cert);
         int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
@@ -396,46 +422,73 @@ __device__ __forceinline__ void stf(void *p, int64_t i, F v) {
     reinterpret_cast<F *>(p)[i] = v;
 }
 
-// Writes one output row; div/shear/vort follow DTFE.cpp:394-433.
-template <typename F, bool VEL, int NS>
-__device__ __forceinline__ void store_row(const OutPtrs &o, int64_t row, const RowAcc<F, NS> &acc, F scale,
-                                          unsigned fields, int S) {
-    if (fields & DTFE_FIELD_DENSITY) stf<F>(o.density, row, acc.d * scale);
-    if (VEL) {
-        if (fields & DTFE_FIELD_VELOCITY)
+// Writes the first `count` of C components of one output row at p[base..].
+// With LPR lanes holding the same row (all with the same values), lane w
+// writes components w, w + LPR, ..., so one store covers LPR components of
+// each row and consecutive rows' components are consecutive addresses.
+template <typename F, int LPR, int C>
+__device__ __forceinline__ void store_field(void *p, int64_t base, const F (&v)[C], int w, int count = C) {
+    F *o = reinterpret_cast<F *>(p) + base;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) stf<F>(o.velocity, 3 * row + c, acc.v[c] * scale);
+    for (int b = 0; b < C; b += LPR) {
+        F x = v[b];
+#pragma unroll
+        for (int c = 1; c < LPR && b + c < C; ++c) x = (w == c) ? v[b + c] : x;
+        if (b + w < count) o[b + w] = x;
+    }
+}
+
+// Writes one output row; div/shear/vort follow DTFE.cpp:394-433.
+// LPR > 1: called by all LPR lanes of the row, w = lane within the row.
+template <typename F, bool VEL, int NS, int LPR = 1>
+__device__ __forceinline__ void store_row(const OutPtrs &o, int64_t row, const RowAcc<F, NS> &acc, F scale,
+                                          unsigned fields, int S, int w = 0) {
+    if (fields & DTFE_FIELD_DENSITY) {
+        const F v[1] = {acc.d * scale};
+        store_field<F, LPR>(o.density, row, v, w);
+    }
+    if (VEL) {
+        if (fields & DTFE_FIELD_VELOCITY) {
+            const F v[3] = {acc.v[0] * scale, acc.v[1] * scale, acc.v[2] * scale};
+            store_field<F, LPR>(o.velocity, 3 * row, v, w);
+        }
         if (fields & (DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE | DTFE_FIELD_VELOCITY_SHEAR |
                       DTFE_FIELD_VELOCITY_VORTICITY)) {
             F g[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) g[i] = acc.vg[i] * scale;
-            if (fields & DTFE_FIELD_VELOCITY_GRADIENT)
-#pragma unroll
-                for (int i = 0; i < 9; ++i) stf<F>(o.velgrad, 9 * row + i, g[i]);
-            if (fields & DTFE_FIELD_VELOCITY_DIVERGENCE) stf<F>(o.div, row, g[0] + g[4] + g[8]);
+            if (fields & DTFE_FIELD_VELOCITY_GRADIENT) store_field<F, LPR>(o.velgrad, 9 * row, g, w);
+            if (fields & DTFE_FIELD_VELOCITY_DIVERGENCE) {
+                const F v[1] = {g[0] + g[4] + g[8]};
+                store_field<F, LPR>(o.div, row, v, w);
+            }
             if (fields & DTFE_FIELD_VELOCITY_SHEAR) {
                 F t0 = (g[0] + g[0]) / (F)2, t1 = (g[1] + g[3]) / (F)2, t2 = (g[2] + g[6]) / (F)2;
                 F t3 = (g[4] + g[4]) / (F)2, t4 = (g[5] + g[7]) / (F)2;
                 F tr = (t0 + t3 + g[8]) / (F)3;
-                stf<F>(o.shear, 5 * row + 0, t0 - tr);
-                stf<F>(o.shear, 5 * row + 1, t1);
-                stf<F>(o.shear, 5 * row + 2, t2);
-                stf<F>(o.shear, 5 * row + 3, t3 - tr);
-                stf<F>(o.shear, 5 * row + 4, t4);
+                const F v[5] = {t0 - tr, t1, t2, t3 - tr, t4};
+                store_field<F, LPR>(o.shear, 5 * row, v, w);
             }
             if (fields & DTFE_FIELD_VELOCITY_VORTICITY) {
-                stf<F>(o.vort, 3 * row + 0, (g[1] - g[3]) / (F)2);
-                stf<F>(o.vort, 3 * row + 1, (g[2] - g[6]) / (F)2);
-                stf<F>(o.vort, 3 * row + 2, (g[5] - g[7]) / (F)2);
+                const F v[3] = {(g[1] - g[3]) / (F)2, (g[2] - g[6]) / (F)2, (g[5] - g[7]) / (F)2};
+                store_field<F, LPR>(o.vort, 3 * row, v, w);
             }
         }
     }
     if (NS > 0) {
-        if (fields & DTFE_FIELD_SCALAR)
-            for (int q = 0; q < S; ++q) stf<F>(o.scalar, (int64_t)S * row + q, acc.s[q] * scale);
-        if (fields & DTFE_FIELD_SCALAR_GRADIENT)
-            for (int q = 0; q < 3 * S; ++q) stf<F>(o.scalargrad, (int64_t)3 * S * row + q, acc.sg[q] * scale);
+        constexpr int C = NS > 0 ? NS : 1;
+        if (fields & DTFE_FIELD_SCALAR) {
+            F v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) v[q] = acc.s[q] * scale;
+            store_field<F, LPR>(o.scalar, (int64_t)S * row, v, w, S);
+        }
+        if (fields & DTFE_FIELD_SCALAR_GRADIENT) {
+            F v[3 * C];
+#pragma unroll
+            for (int q = 0; q < 3 * C; ++q) v[q] = acc.sg[q] * scale;
+            store_field<F, LPR>(o.scalargrad, (int64_t)3 * S * row, v, w, 3 * S);
+        }
     }
 }
 
@@ -449,6 +502,7 @@ struct SeedGrid {
 };
 
 struct SampleArgs {
+    const CellRecord *rec;  // walk records of the cells
     const double *pos;
     const int4 *cells;
     const int4 *nbrs;
@@ -460,6 +514,8 @@ struct SampleArgs {
     int kind, method, M, arg_order;
     int64_t n_rows;
     int64_t n[3];
+    bool rows32;                    // n_rows < 2^32: split rows with FastDiv
+    FastDiv div_n1, div_n2;         // by n[1], n[2]
     const double *ax0, *ax1, *ax2;  // grid: centre coordinates; cone: r, theta, psi
     const double *trig;             // cone: sin(theta)[n1], cos(theta)[n1], sin(psi)[n2], cos(psi)[n2]
     double dx[3];
@@ -470,7 +526,7 @@ struct SampleArgs {
     const uint32_t *plan;           // MT states, one per workgroup
     int64_t rows_per_block;
     int rows_per_round;
-    int ring;                       // MT blocks held in LDS
+    int ring;                       // MT blocks held in LDS (one thread per row)
     OutPtrs out;
     int *status;                    // [0] walk failures, [1] deferred rows, [2] deferral overflow
     int64_t *defer_rows;            // rows the exact kernel redoes
@@ -495,17 +551,34 @@ __device__ __forceinline__ int seed_cell(const SeedGrid &sg, d3 p) {
     return s < 0 ? 0 : s;
 }
 
+// row = (i n1 + j) n2 + k of a grid or cone
+__device__ __forceinline__ void row_ijk(const SampleArgs &a, int64_t row, int64_t &i, int64_t &j, int64_t &k) {
+    if (a.rows32) {
+        const uint32_t r = (uint32_t)row;
+        const uint32_t q = a.div_n2.div(r);
+        const uint32_t ii = a.div_n1.div(q);
+        k = r - q * a.div_n2.d;
+        j = q - ii * a.div_n1.d;
+        i = ii;
+    } else {
+        k = row % a.n[2];
+        j = (row / a.n[2]) % a.n[1];
+        i = row / (a.n[1] * a.n[2]);
+    }
+}
+
+// KIND: a DTFE_SAMPLE_* constant fixed at compile time, or -1 for a.kind
+template <int KIND = -1>
 __device__ __forceinline__ d3 row_centre(const SampleArgs &a, int64_t row) {
-    if (a.kind == DTFE_SAMPLE_GRID) {
-        int64_t k = row % a.n[2];
-        int64_t j = (row / a.n[2]) % a.n[1];
-        int64_t i = row / (a.n[1] * a.n[2]);
+    const int kind = KIND >= 0 ? KIND : a.kind;
+    if (kind == DTFE_SAMPLE_GRID) {
+        int64_t i, j, k;
+        row_ijk(a, row, i, j, k);
         return d3{a.ax0[i], a.ax1[j], a.ax2[k]};
-    } else if (a.kind == DTFE_SAMPLE_CONE) {
+    } else if (kind == DTFE_SAMPLE_CONE) {
 #pragma clang fp contract(off)
-        int64_t k = row % a.n[2];
-        int64_t j = (row / a.n[2]) % a.n[1];
-        int64_t i = row / (a.n[1] * a.n[2]);
+        int64_t i, j, k;
+        row_ijk(a, row, i, j, k);
         double r = a.ax0[i];
         double st = a.trig[j], ct = a.trig[a.n[1] + j];
         double sp = a.trig[2 * a.n[1] + k], cp = a.trig[2 * a.n[1] + a.n[2] + k];
@@ -518,20 +591,21 @@ __device__ __forceinline__ d3 row_centre(const SampleArgs &a, int64_t row) {
 
 __device__ __forceinline__ d3 row_coords(const SampleArgs &a, int64_t row) {
     // the unconverted cell coordinates (grid x,y,z or cone r,theta,psi)
-    int64_t k = row % a.n[2];
-    int64_t j = (row / a.n[2]) % a.n[1];
-    int64_t i = row / (a.n[1] * a.n[2]);
+    int64_t i, j, k;
+    row_ijk(a, row, i, j, k);
     return d3{a.ax0[i], a.ax1[j], a.ax2[k]};
 }
 
 // MC sample point from three uniforms u[0..2] (in stream order).
+template <int KIND = -1>
 __device__ __forceinline__ d3 mc_point(const SampleArgs &a, int64_t row, d3 centre, const double u[3]) {
-    if (a.kind == DTFE_SAMPLE_GRID) {
+    const int kind = KIND >= 0 ? KIND : a.kind;
+    if (kind == DTFE_SAMPLE_GRID) {
         double ux = a.arg_order == 0 ? u[0] : u[2];
         double uz = a.arg_order == 0 ? u[2] : u[0];
         return d3{mul_add_rn(ux, a.dx[0], centre.x), mul_add_rn(u[1], a.dx[1], centre.y),
                   mul_add_rn(uz, a.dx[2], centre.z)};
-    } else if (a.kind == DTFE_SAMPLE_CONE) {
+    } else if (kind == DTFE_SAMPLE_CONE) {
 #pragma clang fp contract(off)
         d3 c = row_coords(a, row);
         double tr = c.x + u[0] * a.dx[0];
@@ -548,14 +622,19 @@ __device__ __forceinline__ d3 mc_point(const SampleArgs &a, int64_t row, d3 cent
 
 // Source of the row's Monte-Carlo words: the LDS ring of the MC kernel or
 // the words saved with a deferred row.
+// NSLOT > 0: ring size fixed at compile time; 0: runtime n.
+// Offsets from the first word of slot 0 stay below 2^32 (a workgroup spans
+// at most rows_per_block * 3M words).
+template <int NSLOT = 0>
 struct RingWords {
     const uint32_t *ring;
-    int64_t base;  // MT block held in slot 0
-    int n;         // slots
+    int64_t word0;  // stream index of the first word of the block in slot 0
+    uint32_t n;     // slots (NSLOT = 0)
     __device__ __forceinline__ uint32_t operator()(int64_t word) const {
-        int64_t blk = word / MT_N;
-        int slot = (int)(blk - base) % n;
-        return ring[slot * MT_N + (int)(word - blk * MT_N)];
+        const uint32_t rel = (uint32_t)(word - word0);
+        const uint32_t blk = rel / (uint32_t)MT_N;
+        const uint32_t slot = NSLOT > 0 ? blk % (uint32_t)NSLOT : blk % n;
+        return ring[slot * MT_N + (rel - blk * MT_N)];
     }
 };
 
@@ -572,7 +651,7 @@ __device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, co
     const bool value_bug = (a.flags & DTFE_COMPAT_VALUE_BUG) != 0;
     d3 centre = row_centre(a, row);
     int start = seed_cell(a.sg, centre);
-    Located Lc = locate<EXACT>(a.pos, a.cells, a.nbrs, centre, start);
+    Located Lc = locate<EXACT>(a.rec, centre, start);
     if (!EXACT && Lc.t == -3) return false;
 #ifdef DTFE_COUNT_STEPS
     int sample_steps = 0;
@@ -605,7 +684,7 @@ __device__ __forceinline__ bool process_row(const SampleArgs &a, int64_t row, co
             for (int q = 0; q < 3; ++q) u[q] = mt_uniform(mt_temper(words(word + q)));
             p = mc_point(a, row, centre, u);
         }
-        Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, p, hint);
+        Located L = locate<EXACT>(a.rec, p, hint);
         if (!EXACT && L.t == -3) return false;
 #ifdef DTFE_COUNT_STEPS
         sample_steps += L.steps;
@@ -825,7 +904,7 @@ __device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, co
     if (!g.any_g) return true;
     d3 centre = row_centre(a, row);
     int start = seed_cell(a.sg, centre);
-    Located Lc = locate<EXACT>(a.pos, a.cells, a.nbrs, centre, start);
+    Located Lc = locate<EXACT>(a.rec, centre, start);
     if (!EXACT && Lc.t == -3) return false;
     if (Lc.t == -2) {
         atomicAdd(a.status, 1);
@@ -850,14 +929,14 @@ __device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, co
     if (!EXACT) {
         // make sure every walk of the row is certified before scattering anything
         for (int w = 0; w < a.M; ++w) {
-            Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, sample_point(w), hint);
+            Located L = locate<EXACT>(a.rec, sample_point(w), hint);
             if (L.t == -3) return false;
         }
     }
     const F s = (F)1 / (F)a.M;
     for (int w = 0; w < a.M; ++w) {
         d3 p = sample_point(w);
-        Located L = locate<EXACT>(a.pos, a.cells, a.nbrs, p, hint);
+        Located L = locate<EXACT>(a.rec, p, hint);
         if (L.t == -2) {
             atomicAdd(a.status, 1);
             continue;
@@ -900,26 +979,167 @@ __device__ __forceinline__ void mt_generate_block(uint32_t *ring, int64_t q, int
     __syncthreads();
     const uint32_t *old = ring + (int)((q - 1 - base) % n) * MT_N;
     uint32_t *nw = ring + (int)((q - base) % n) * MT_N;
-    const int t = threadIdx.x;
-    if (t < 227) {
+    for (int t = threadIdx.x; t < 227; t += blockDim.x) {
         uint32_t y = (old[t] & MT_UPPER) | (old[t + 1] & MT_LOWER);
         uint32_t v = old[t + MT_M] ^ (y >> 1);
         nw[t] = (y & 1u) ? (v ^ MT_MATRIX) : v;
     }
     __syncthreads();
-    if (t < 227) {
+    for (int t = threadIdx.x; t < 227; t += blockDim.x) {
         int i = t + 227;
         uint32_t y = (old[i] & MT_UPPER) | (old[i + 1] & MT_LOWER);
         uint32_t v = nw[i - 227] ^ (y >> 1);
         nw[i] = (y & 1u) ? (v ^ MT_MATRIX) : v;
     }
     __syncthreads();
-    if (t < MT_N - 454) {
+    for (int t = threadIdx.x; t < MT_N - 454; t += blockDim.x) {
         int i = t + 454;
         uint32_t nxt = (i == MT_N - 1) ? nw[0] : old[i + 1];
         uint32_t y = (old[i] & MT_UPPER) | (nxt & MT_LOWER);
         uint32_t v = nw[i - 227] ^ (y >> 1);
         nw[i] = (y & 1u) ? (v ^ MT_MATRIX) : v;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Monte-Carlo rows as M lanes: (A) locate the cell centres, (B) one thread per
+// sample walking from its cell's centre tetrahedron, cell sums by shuffles
+// ----------------------------------------------------------------------------
+// centre[row]: located tetrahedron, -1 outside the hull, -3 uncertified sign
+// (the row then goes to the exact kernel from pass B, which holds its words).
+template <int KIND>
+__global__ void __launch_bounds__(kBlock) k_locate_centres(SampleArgs a, int32_t *__restrict__ centre) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride) {
+        d3 c = row_centre<KIND>(a, row);
+        Located L = locate<false>(a.rec, c, seed_cell(a.sg, c));
+        int t = L.t;
+        if (t == -2) {
+            atomicAdd(a.status, 1);
+            t = -1;
+        }
+        centre[row] = t;
+        if (a.out.cell && t != -3) a.out.cell[row] = t;
+#ifdef DTFE_COUNT_STEPS
+        atomicAdd((unsigned long long *)(a.status + 4), (unsigned long long)L.steps);
+#endif
+    }
+}
+
+template <int LPR, typename T>
+__device__ __forceinline__ T lane_sum(T v) {
+#pragma unroll
+    for (int o = 1; o < LPR; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename F, bool VEL, int NS, int LPR, int KIND>
+__global__ void __launch_bounds__(kLanesBlock, DTFE_LANES_MIN_WAVES) k_sample_mc_lanes(SampleArgs a, const int32_t *__restrict__ centre) {
+    extern __shared__ uint32_t ring[];
+    constexpr int64_t M = LPR;
+    constexpr int64_t wpr = 3 * M;
+    constexpr int rows_per_round = kLanesBlock / LPR;
+    const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
+    if (row0 >= a.n_rows) return;
+    const int64_t row_end = min(row0 + a.rows_per_block, a.n_rows);
+    const int64_t first_block = (wpr * row0) / MT_N;
+    const int64_t base = first_block - 1;
+    for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
+    RingWords<kRingLanes> words{ring, base * MT_N, (uint32_t)kRingLanes};
+    int64_t have = first_block - 1;
+    const int w = threadIdx.x % LPR;
+    const bool value_bug = (a.flags & DTFE_COMPAT_VALUE_BUG) != 0;
+    const bool want_std = VEL && KIND == DTFE_SAMPLE_GRID && (a.fields & DTFE_FIELD_VELOCITY_STD);
+    for (int64_t r0 = row0; r0 < row_end; r0 += rows_per_round) {
+        int64_t r1 = min(r0 + rows_per_round, row_end);
+        int64_t need = (wpr * r1 - 1) / MT_N;
+        while (have < need) {
+            mt_generate_block(ring, have + 1, base, kRingLanes);
+            ++have;
+        }
+        __syncthreads();
+        const int64_t row = r0 + threadIdx.x / LPR;
+        const bool active = row < r1;  // uniform over the LPR lanes of a row
+        int unsure = 0;
+        RowAcc<F, NS> acc;
+        acc.clear();
+        F vs[3] = {0, 0, 0};  // this sample's velocity, for the standard deviation
+        bool finite = false;
+        if (active) {
+            const d3 c = row_centre<KIND>(a, row);
+            int start = centre[row];
+            if (start == -3) unsure = 1;
+            if (start < 0) start = seed_cell(a.sg, c);
+            const int64_t word = 3 * (M * row + w);
+            double u[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) u[q] = mt_uniform(mt_temper(words(word + q)));
+            const d3 p = mc_point<KIND>(a, row, c, u);
+            if (!unsure) {
+                Located L = locate<false>(a.rec, p, start);
+#ifdef DTFE_COUNT_STEPS
+                atomicAdd((unsigned long long *)(a.status + 6), (unsigned long long)L.steps);
+#endif
+                if (L.t == -3) unsure = 1;
+                else if (L.t == -2) atomicAdd(a.status, 1);
+                else if (L.t >= 0) {
+                    finite = true;
+                    // the std needs this sample's velocity value (the reference's velocityValue)
+                    const unsigned f = a.fields | (want_std ? DTFE_FIELD_VELOCITY : 0u);
+                    add_sample<F, VEL, NS>(L, p, a.vf, f, value_bug, acc, false);
+                    if (want_std) {
+                        vs[0] = acc.v[0];
+                        vs[1] = acc.v[1];
+                        vs[2] = acc.v[2];
+                    }
+                }
+            }
+        }
+        // a row with an uncertified sign goes, whole, to the exact kernel
+        unsure = lane_sum<LPR>(unsure);
+        if (active && unsure) {
+            int slot = 0;
+            if (w == 0) {
+                slot = atomicAdd(a.status + 1, 1);
+                if (slot < a.defer_cap) a.defer_rows[slot] = row;
+                else atomicOr(a.status + 2, 1);
+            }
+            slot = __shfl(slot, (int)(threadIdx.x & 63) - w, 64);
+            if (slot < a.defer_cap) {
+                const int64_t word = 3 * (M * row + w);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) a.defer_words[(int64_t)slot * wpr + 3 * w + q] = words(word + q);
+            }
+        }
+        // cell sums over the row's lanes (all lanes take part in the shuffles)
+        int nfin = lane_sum<LPR>(finite ? 1 : 0);
+        acc.d = lane_sum<LPR>(acc.d);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc.v[q] = lane_sum<LPR>(acc.v[q]);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc.vg[q] = lane_sum<LPR>(acc.vg[q]);
+#pragma unroll
+        for (int q = 0; q < (NS > 0 ? NS : 1); ++q) acc.s[q] = lane_sum<LPR>(acc.s[q]);
+#pragma unroll
+        for (int q = 0; q < 3 * (NS > 0 ? NS : 1); ++q) acc.sg[q] = lane_sum<LPR>(acc.sg[q]);
+        F sd = 0;
+        if (want_std) {
+            // the reference's two-pass standard deviation over the M values
+            F mean[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) mean[q] = lane_sum<LPR>(vs[q]) / (F)M;
+            F d2 = 0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) d2 += (vs[q] - mean[q]) * (vs[q] - mean[q]);
+            sd = sqrt(lane_sum<LPR>(d2) / (F)M);
+        }
+        if (active && !unsure) {
+            F scale;
+            if (a.flags & DTFE_AVERAGE_FINITE_ONLY) scale = nfin > 0 ? (F)1 / (F)nfin : (F)0;
+            else scale = (F)1 / (F)M;
+            store_row<F, VEL, NS, LPR>(a.out, row, acc, scale, a.fields, a.vf.S, w);
+            if (want_std && w == 0) stf<F>(a.out.velstd, row, sd);
+        }
     }
 }
 
@@ -946,7 +1166,7 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_mc(SampleArgs a) {
     // slot 0 holds the plan state, i.e. MT block first_block - 1
     const int64_t base = first_block - 1;
     for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
-    RingWords words{ring, base, a.ring};
+    RingWords<> words{ring, base * MT_N, (uint32_t)a.ring};
     int64_t have = first_block - 1;  // last generated block
     for (int64_t r0 = row0; r0 < row_end; r0 += a.rows_per_round) {
         int64_t r1 = min(r0 + a.rows_per_round, row_end);
@@ -991,7 +1211,7 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_exact_all(SampleArgs a) {
     const int64_t first_block = (wpr * row0) / MT_N;
     const int64_t base = first_block - 1;
     for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
-    RingWords words{ring, base, a.ring};
+    RingWords<> words{ring, base * MT_N, (uint32_t)a.ring};
     int64_t have = first_block - 1;
     for (int64_t r0 = row0; r0 < row_end; r0 += a.rows_per_round) {
         int64_t r1 = min(r0 + a.rows_per_round, row_end);
@@ -1009,26 +1229,58 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_exact_all(SampleArgs a) {
 // ----------------------------------------------------------------------------
 // vertex density
 // ----------------------------------------------------------------------------
-__global__ void k_cell_volumes(const double *__restrict__ pos, const int4 *__restrict__ cells,
-                               const int4 *__restrict__ nbrs, int64_t nc, double *__restrict__ W,
-                               uint8_t *__restrict__ hull) {
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
-        int4 c = cells[t];
-        d3 p0 = load_pos(pos, c.x), p1 = load_pos(pos, c.y), p2 = load_pos(pos, c.z), p3 = load_pos(pos, c.w);
-        d3 a = sub(p1, p0), b = sub(p2, p0), d = sub(p3, p0);
-        double det = a.x * (b.y * d.z - b.z * d.y) - a.y * (b.x * d.z - b.z * d.x) + a.z * (b.x * d.y - b.y * d.x);
-        double vol = det / 6.0;
-        atomicAdd(W + c.x, vol);
-        atomicAdd(W + c.y, vol);
-        atomicAdd(W + c.z, vol);
-        atomicAdd(W + c.w, vol);
-        int4 nb = nbrs[t];
-        // a face on the hull: its three vertices touch an infinite cell
-        if (nb.x < 0) { hull[c.y] = 1; hull[c.z] = 1; hull[c.w] = 1; }
-        if (nb.y < 0) { hull[c.x] = 1; hull[c.z] = 1; hull[c.w] = 1; }
-        if (nb.z < 0) { hull[c.x] = 1; hull[c.y] = 1; hull[c.w] = 1; }
-        if (nb.w < 0) { hull[c.x] = 1; hull[c.y] = 1; hull[c.z] = 1; }
+// One thread per tetrahedron. The 4 (vertex, volume) pairs of a workgroup's
+// 256 tetrahedra are first merged in an LDS hash table (Morton-ordered
+// meshes make them share ~4x fewer distinct vertices), then each distinct
+// vertex gets one FP64 atomic in HBM.
+constexpr int kAggSlots = 2048;  // > 4 * kBlock, load factor <= 0.5
+
+__global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restrict__ pos,
+                                                         const int4 *__restrict__ cells,
+                                                         const int4 *__restrict__ nbrs, int64_t nc,
+                                                         double *__restrict__ W, uint8_t *__restrict__ hull) {
+    __shared__ int keys[kAggSlots];
+    __shared__ double vals[kAggSlots];
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
+        for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+            keys[i] = -1;
+            vals[i] = 0.0;
+        }
+        __syncthreads();
+        const int64_t t = base + threadIdx.x;
+        if (t < nc) {
+            int4 c = cells[t];
+            d3 p0 = load_pos(pos, c.x), p1 = load_pos(pos, c.y), p2 = load_pos(pos, c.z), p3 = load_pos(pos, c.w);
+            d3 a = sub(p1, p0), b = sub(p2, p0), d = sub(p3, p0);
+            double det = a.x * (b.y * d.z - b.z * d.y) - a.y * (b.x * d.z - b.z * d.x) + a.z * (b.x * d.y - b.y * d.x);
+            double vol = det / 6.0;
+            const int id[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int v = id[q];
+                unsigned slot = ((unsigned)v * 2654435761u) >> 21;  // 11 bits
+                for (int probe = 0; probe < kAggSlots; ++probe) {
+                    int k = atomicCAS(&keys[slot], -1, v);
+                    if (k == -1 || k == v) {
+                        atomicAdd(&vals[slot], vol);
+                        break;
+                    }
+                    slot = (slot + 1) & (kAggSlots - 1);
+                }
+            }
+            int4 nb = nbrs[t];
+            // a face on the hull: its three vertices touch an infinite cell
+            if (nb.x < 0) { hull[c.y] = 1; hull[c.z] = 1; hull[c.w] = 1; }
+            if (nb.y < 0) { hull[c.x] = 1; hull[c.z] = 1; hull[c.w] = 1; }
+            if (nb.z < 0) { hull[c.x] = 1; hull[c.y] = 1; hull[c.w] = 1; }
+            if (nb.w < 0) { hull[c.x] = 1; hull[c.y] = 1; hull[c.z] = 1; }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+            int k = keys[i];
+            if (k >= 0) atomicAdd(W + k, vals[i]);
+        }
+        __syncthreads();
     }
 }
 
@@ -1310,6 +1562,34 @@ __global__ void k_deposit_finish(const F *__restrict__ mass, const F *__restrict
     }
 }
 
+// Cell records, eight lanes per record: lane j writes 16 bytes (j < 6: two of
+// the twelve coordinates, 6: vertex ids, 7: neighbours), so each wave stores
+// eight whole records contiguously.
+__device__ __forceinline__ int vertex_of(int4 c, int q) { return q == 0 ? c.x : q == 1 ? c.y : q == 2 ? c.z : c.w; }
+
+__global__ void __launch_bounds__(kBlock) k_cell_records(const double *__restrict__ pos, const int4 *__restrict__ cells,
+                                                         const int4 *__restrict__ nbrs, int64_t nc,
+                                                         int4 *__restrict__ rec) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < 8 * nc; g += stride) {
+        const int64_t t = g >> 3;
+        const int j = (int)(g & 7);
+        int4 out;
+        if (j < 6) {
+            const int4 c = cells[t];
+            const int k0 = 2 * j, k1 = 2 * j + 1;
+            const double xy[2] = {pos[3 * (int64_t)vertex_of(c, k0 / 3) + k0 % 3],
+                                  pos[3 * (int64_t)vertex_of(c, k1 / 3) + k1 % 3]};
+            __builtin_memcpy(&out, xy, sizeof(out));
+        } else if (j == 6) {
+            out = cells[t];
+        } else {
+            out = nbrs[t];
+        }
+        rec[g] = out;
+    }
+}
+
 template <typename T>
 __global__ void k_fill(T *p, int64_t n, T v) {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1338,13 +1618,14 @@ struct dtfe_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf W, hull, seed_a, seed_b, bbox_part, bbox, status, axes, trig, equi, plan, flags, rows, mass, mom,
-        defer_rows, defer_words, adj_rho, adj_dW, adj_partial;
+        defer_rows, defer_words, adj_rho, adj_dW, adj_partial, centre, records;
     int last_deferred = 0;
     int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples
     // HIP events around the main kernels of the last calls (on ctx->stream)
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};
     bool ev_pending[4] = {false, false, false, false};
-    double last_ms[4] = {0, 0, 0, 0};  // 0 sample main kernel, 1 exact kernel, 2 seed grid, 3 vertex density
+    // 0 sample main kernel, 1 exact kernel, 2 seed grid, 3 vertex density, 4 centre location
+    double last_ms[5] = {0, 0, 0, 0, 0};
     // MC plan cache
     bool plan_valid = false;
     uint32_t plan_seed = 0;
@@ -1396,6 +1677,29 @@ int check_mesh(dtfe_ctx *ctx, const dtfe_mesh *m) {
         return fail(ctx, DTFE_ERR_ARGUMENT, "mesh larger than 2^31 vertices or cells");
     if (m->n_vertices >= 4 && m->n_cells > 0 && (!m->pos || !m->cells || !m->neighbors))
         return fail(ctx, DTFE_ERR_ARGUMENT, "mesh arrays are NULL");
+    return DTFE_OK;
+}
+
+int launch_cell_records(dtfe_ctx *ctx, const dtfe_mesh *m, void *records) {
+    if (m->n_cells == 0) return DTFE_OK;
+    hipLaunchKernelGGL(k_cell_records, dim3(grid_for(8 * m->n_cells)), dim3(kBlock), 0, ctx->stream, m->pos,
+                       (const int4 *)m->cells, (const int4 *)m->neighbors, m->n_cells, (int4 *)records);
+    HIP_TRY(hipGetLastError());
+    return DTFE_OK;
+}
+
+// The walk records: the caller's, or built for this call.
+int use_cell_records(dtfe_ctx *ctx, const dtfe_mesh *m, SampleArgs &a) {
+    if (m->cell_records) {
+        if ((uintptr_t)m->cell_records % 16)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "cell_records must be 16-byte aligned");
+        a.rec = (const CellRecord *)m->cell_records;
+        return DTFE_OK;
+    }
+    int rc;
+    if ((rc = ensure(ctx, ctx->records, (size_t)m->n_cells * sizeof(CellRecord)))) return rc;
+    if ((rc = launch_cell_records(ctx, m, ctx->records.p))) return rc;
+    a.rec = (const CellRecord *)ctx->records.p;
     return DTFE_OK;
 }
 
@@ -1458,6 +1762,41 @@ void launch_ns(const SampleArgs &a, int ns, bool adj, int which, int mc, int gri
         case 2: launch_sample<F, VEL, 2, false>(a, which, mc, grid, lds, n_def, s); break;
         case 3: launch_sample<F, VEL, 3, false>(a, which, mc, grid, lds, n_def, s); break;
         default: launch_sample<F, VEL, 4, false>(a, which, mc, grid, lds, n_def, s); break;
+    }
+}
+
+template <typename F, bool VEL, int NS, int KIND>
+void launch_lanes_lpr(const SampleArgs &a, const int32_t *centre, int lpr, int grid, size_t lds, hipStream_t s) {
+    switch (lpr) {
+        case 2: hipLaunchKernelGGL((k_sample_mc_lanes<F, VEL, NS, 2, KIND>), dim3(grid), dim3(kLanesBlock), lds, s, a, centre); break;
+        case 4: hipLaunchKernelGGL((k_sample_mc_lanes<F, VEL, NS, 4, KIND>), dim3(grid), dim3(kLanesBlock), lds, s, a, centre); break;
+        case 8: hipLaunchKernelGGL((k_sample_mc_lanes<F, VEL, NS, 8, KIND>), dim3(grid), dim3(kLanesBlock), lds, s, a, centre); break;
+        default: hipLaunchKernelGGL((k_sample_mc_lanes<F, VEL, NS, 16, KIND>), dim3(grid), dim3(kLanesBlock), lds, s, a, centre); break;
+    }
+}
+
+// scalars: none, or up to kMaxScalars with the count read at run time
+template <typename F, bool VEL, int KIND>
+void launch_lanes_ns(const SampleArgs &a, const int32_t *centre, int ns, int lpr, int grid, size_t lds, hipStream_t s) {
+    if (ns == 0) launch_lanes_lpr<F, VEL, 0, KIND>(a, centre, lpr, grid, lds, s);
+    else launch_lanes_lpr<F, VEL, kMaxScalars, KIND>(a, centre, lpr, grid, lds, s);
+}
+
+template <typename F, bool VEL>
+void launch_lanes(const SampleArgs &a, const int32_t *centre, int ns, int lpr, int grid, size_t lds, hipStream_t s) {
+    switch (a.kind) {
+        case DTFE_SAMPLE_GRID: launch_lanes_ns<F, VEL, DTFE_SAMPLE_GRID>(a, centre, ns, lpr, grid, lds, s); break;
+        case DTFE_SAMPLE_CONE: launch_lanes_ns<F, VEL, DTFE_SAMPLE_CONE>(a, centre, ns, lpr, grid, lds, s); break;
+        default: launch_lanes_ns<F, VEL, DTFE_SAMPLE_POINTS>(a, centre, ns, lpr, grid, lds, s); break;
+    }
+}
+
+void launch_centres(const SampleArgs &a, int32_t *centre, hipStream_t s) {
+    const dim3 g(grid_for(a.n_rows)), b(kBlock);
+    switch (a.kind) {
+        case DTFE_SAMPLE_GRID: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_GRID>, g, b, 0, s, a, centre); break;
+        case DTFE_SAMPLE_CONE: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_CONE>, g, b, 0, s, a, centre); break;
+        default: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_POINTS>, g, b, 0, s, a, centre); break;
     }
 }
 
@@ -1553,6 +1892,9 @@ int setup_rows(dtfe_ctx *ctx, const dtfe_sampling *smp, int method, SampleArgs &
         return fail(ctx, DTFE_ERR_ARGUMENT, "unknown sampling kind");
     }
     a.n_rows = n_rows;
+    a.rows32 = n_rows < ((int64_t)1 << 32);
+    a.div_n1 = FastDiv((uint32_t)a.n[1]);
+    a.div_n2 = FastDiv((uint32_t)a.n[2]);
     return DTFE_OK;
 }
 
@@ -1610,6 +1952,9 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     int grid;
     size_t lds = 0;
     int is_mc = a.method == DTFE_METHOD_MC;
+    // one thread per sample when M is a power of two in [2, 16] (forward only)
+    const bool lanes = is_mc && !adj && (a.M == 2 || a.M == 4 || a.M == 8 || a.M == 16);
+    size_t lds_lanes = 0;
     if (is_mc) {
         const int64_t wpr = 3 * (int64_t)a.M;
         int rpr = kBlock;
@@ -1620,6 +1965,14 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
         lds = (size_t)a.ring * MT_N * sizeof(uint32_t);
         int64_t rounds = std::max<int64_t>(1, std::min<int64_t>(16, n_rows / ((int64_t)rpr * 2048)));
         a.rows_per_block = rpr * rounds;
+        if (lanes) {
+            const int64_t rows_lanes = kLanesBlock / a.M;  // rows per round with one thread per sample
+            lds_lanes = (size_t)kRingLanes * MT_N * sizeof(uint32_t);
+            // keep a.rows_per_block a multiple of both round sizes
+            int64_t rl = std::max<int64_t>(1, std::min<int64_t>(64, n_rows / (rows_lanes * 4096)));
+            a.rows_per_block = std::max<int64_t>(rows_lanes * rl, rpr);
+            a.rows_per_block = ((a.rows_per_block + rpr - 1) / rpr) * rpr;
+        }
         int64_t nblk = (n_rows + a.rows_per_block - 1) / a.rows_per_block;
         grid = (int)nblk;
         uint64_t s = mc->seed;
@@ -1654,7 +2007,20 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     a.defer_words = (uint32_t *)ctx->defer_words.p;
     a.defer_cap = (int)cap;
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    launch_any(a, f64, vel, ns, adj, 0, is_mc, grid, lds, 0, ctx->stream);
+    if (lanes) {
+        if ((rc = ensure(ctx, ctx->centre, n_rows * sizeof(int32_t)))) return rc;
+        launch_centres(a, (int32_t *)ctx->centre.p, ctx->stream);
+        HIP_TRY(hipEventRecord(ctx->ev[8], ctx->stream));
+        if (f64) {
+            if (vel) launch_lanes<double, true>(a, (const int32_t *)ctx->centre.p, ns, a.M, grid, lds_lanes, ctx->stream);
+            else launch_lanes<double, false>(a, (const int32_t *)ctx->centre.p, ns, a.M, grid, lds_lanes, ctx->stream);
+        } else {
+            if (vel) launch_lanes<float, true>(a, (const int32_t *)ctx->centre.p, ns, a.M, grid, lds_lanes, ctx->stream);
+            else launch_lanes<float, false>(a, (const int32_t *)ctx->centre.p, ns, a.M, grid, lds_lanes, ctx->stream);
+        }
+    } else {
+        launch_any(a, f64, vel, ns, adj, 0, is_mc, grid, lds, 0, ctx->stream);
+    }
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
     HIP_TRY(hipGetLastError());
     int st[8];
@@ -1670,8 +2036,17 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
         ctx->last_walk[3] = n_rows * (int64_t)(a.method == DTFE_METHOD_CENTRE ? 0 : a.M);
     }
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    ctx->last_ms[0] = ms;
+    if (lanes) {
+        float mc_ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[8]));
+        HIP_TRY(hipEventElapsedTime(&mc_ms, ctx->ev[8], ctx->ev[1]));
+        ctx->last_ms[4] = ms;
+        ctx->last_ms[0] = mc_ms;
+    } else {
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->last_ms[0] = ms;
+        ctx->last_ms[4] = 0.0;
+    }
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
     ctx->last_ms[2] = ms;
     ctx->last_ms[1] = 0.0;
@@ -1717,7 +2092,7 @@ int dtfe_hip_create(int device, dtfe_ctx **out) {
         return DTFE_ERR_HIP;
     }
     ctx->stream = ctx->own;
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 9; ++i)
         if (hipEventCreate(&ctx->ev[i]) != hipSuccess) {
             delete ctx;
             return DTFE_ERR_HIP;
@@ -1733,10 +2108,10 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_a, &ctx->seed_b, &ctx->bbox_part, &ctx->bbox,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
-                      &ctx->adj_partial};
+                      &ctx->adj_partial, &ctx->centre, &ctx->records};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 9; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -1756,6 +2131,18 @@ int dtfe_hip_synchronize(dtfe_ctx *ctx) {
 }
 
 const char *dtfe_hip_last_error(dtfe_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int dtfe_hip_cell_records(dtfe_ctx *ctx, const dtfe_mesh *m, void *records) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    int rc = check_mesh(ctx, m);
+    if (rc) return rc;
+    if (m->n_cells > 0 && (!m->pos || !m->cells || !m->neighbors))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "mesh arrays are NULL");
+    if (m->n_cells > 0 && !records) return fail(ctx, DTFE_ERR_ARGUMENT, "records is NULL");
+    if ((uintptr_t)records % 16) return fail(ctx, DTFE_ERR_ARGUMENT, "records must be 16-byte aligned");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return launch_cell_records(ctx, m, records);
+}
 
 int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *m, int dtype, const void *weight, double rho_bar,
                             void *rho_out) {
@@ -1887,6 +2274,7 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
                                out->located_cell, n_rows, -1);
         return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: fields are zero");
     }
+    if ((rc = use_cell_records(ctx, m, a))) return rc;
     if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
     const int ns = want_scal ? vf->n_scalar : 0;
     return launch_rows(ctx, a, mc, vf->dtype == DTFE_F64, want_vel, ns, false);
@@ -2026,6 +2414,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     std::vector<double> axes, trig;
     if ((rc = setup_rows(ctx, smp, method, a, axes, trig))) return rc;
     if (a.n_rows > 0) {
+        if ((rc = use_cell_records(ctx, m, a))) return rc;
         if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
         if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true))) return rc;
     }
@@ -2093,7 +2482,7 @@ int dtfe_hip_last_timings(dtfe_ctx *ctx, double *ms) {
         ctx->last_ms[3] = t;
         ctx->ev_pending[3] = false;
     }
-    for (int i = 0; i < 4; ++i) ms[i] = ctx->last_ms[i];
+    for (int i = 0; i < 5; ++i) ms[i] = ctx->last_ms[i];
     return DTFE_OK;
 }
 

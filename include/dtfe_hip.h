@@ -79,7 +79,15 @@ typedef struct {
     const double  *pos;        /* [n_vertices][3] */
     const int32_t *cells;      /* [n_cells][4] */
     const int32_t *neighbors;  /* [n_cells][4] */
+    /* Optional: [n_cells] records of DTFE_CELL_RECORD_BYTES filled by
+     * dtfe_hip_cell_records from this pos/cells/neighbors (the point-location
+     * walk reads one record per tetrahedron). NULL: each sampling call builds
+     * them itself. They copy the positions: rebuild after changing pos. */
+    const void *cell_records;
 } dtfe_mesh;
+
+/* Bytes per cell record: 4 FP64 vertex positions, 4 vertex ids, 4 neighbours. */
+#define DTFE_CELL_RECORD_BYTES 128
 
 typedef struct {
     int dtype;               /* DTFE_F32 or DTFE_F64: type of every array below */
@@ -130,6 +138,13 @@ int dtfe_hip_destroy(dtfe_ctx *ctx);
 int dtfe_hip_set_stream(dtfe_ctx *ctx, void *hip_stream);
 int dtfe_hip_synchronize(dtfe_ctx *ctx);
 const char *dtfe_hip_last_error(dtfe_ctx *ctx);
+
+/* Fills `records` (device memory, n_cells * DTFE_CELL_RECORD_BYTES, 16-byte
+ * aligned) from the mesh's pos/cells/neighbors: the device-resident form of
+ * the triangulation's cells, each with its vertex points, as the reference's
+ * CGAL cells reach them (src/CGAL_triangulation/triangulation.cpp:162-195).
+ * Build once per mesh and pass it as dtfe_mesh.cell_records. */
+int dtfe_hip_cell_records(dtfe_ctx *ctx, const dtfe_mesh *mesh, void *records);
 
 /* Vertex density, replacing vertexDensity() (src/CGAL_triangulation/triangulation.cpp:200-279):
  * W_v = sum of the volumes of the finite tetrahedra around v,
@@ -190,9 +205,11 @@ const char *dtfe_hip_version(void);
 
 /* ---- diagnostics (no counterpart in the reference) ------------------------------ */
 /* Kernel times of the last calls, from HIP events on the context's stream
- * (synchronises it): ms[0] main sampling kernel, ms[1] exact-sign kernel,
- * ms[2] seed-grid build, ms[3] vertex density. */
-int dtfe_hip_last_timings(dtfe_ctx *ctx, double ms[4]);
+ * (synchronises it): ms[0] main sampling kernel (the per-sample kernel of
+ * the Monte-Carlo lanes path), ms[1] exact-sign kernel, ms[2] seed-grid
+ * build, ms[3] vertex density, ms[4] cell-centre location of the lanes path
+ * (0 otherwise). */
+int dtfe_hip_last_timings(dtfe_ctx *ctx, double ms[5]);
 /* Rows of the last dtfe_hip_sample call that needed exact signs. */
 int dtfe_hip_last_deferred(dtfe_ctx *ctx);
 /* Point locations of the last dtfe_hip_sample call that did not terminate. */

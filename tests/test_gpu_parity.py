@@ -141,6 +141,25 @@ def test_grid_monte_carlo_many_blocks(gpu):
         assert e < TOL64
 
 
+@pytest.mark.parametrize("samples", [2, 4, 16])
+def test_grid_monte_carlo_sample_lanes(gpu, samples):
+    """Powers of two up to 16 run one thread per sample (cell sums across
+    lanes); many workgroups, each starting from its own MT plan state."""
+    from dtfe_amd import api
+    prob = P.uniform_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float64)
+    n = (64, 64, 40)
+    fields = ["density", "velocity", "velocity_gradient", "velocity_std"]
+    out = api.sample(dm, rho, vel, None, fields, api.GridSpec(prob.region, n), method=2,
+                     samples=samples, seed=4242, part_no=1)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="grid",
+                   box=prob.region, n=n, method=2, samples=samples, seed=4242, part_no=1)
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"MC lanes M={samples} {f}: {e:.3e}")
+        assert e < TOL64, f
+
+
 def test_grid_equidistant(gpu):
     from dtfe_amd import api
     prob = P.clustered_problem()
@@ -175,8 +194,8 @@ def test_cone(gpu, method):
         assert e < (TOL64 if method == 0 else 1e-9)
 
 
-@pytest.mark.parametrize("method", [0, 2])
-def test_user_points(gpu, method):
+@pytest.mark.parametrize("method,samples", [(0, 1), (2, 6), (2, 8)])
+def test_user_points(gpu, method, samples):
     from dtfe_amd import api
     prob = P.clustered_problem()
     dm, rho, vel, sc = gpu_setup(prob, torch.float64)
@@ -187,9 +206,9 @@ def test_user_points(gpu, method):
     fields = ["density", "velocity", "velocity_gradient"]
     out = api.sample(dm, rho, vel, None, fields,
                      api.PointsSpec(torch.from_numpy(pts).cuda(), torch.from_numpy(deltas).cuda(), box),
-                     method=method, samples=6, seed=77)
+                     method=method, samples=samples, seed=77)
     ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="points", box=box,
-                   points=pts, deltas=deltas, method=method, samples=6, seed=77)
+                   points=pts, deltas=deltas, method=method, samples=samples, seed=77)
     assert out["density"].shape[0] == ref["density"].shape[0]
     for f in fields:
         e = rel_err(out[f].cpu().numpy(), ref[f])
@@ -269,3 +288,55 @@ def test_derived_fields_without_gradient(gpu, method):
         e = rel_err(out[f].cpu().numpy(), ref[f])
         print(f"derived only method={method} {f}: {e:.3e}")
         assert e < TOL64, f
+
+
+def test_cell_records_layout_and_use(gpu):
+    """Cell records hold each tetrahedron's four FP64 positions, vertex ids
+    and neighbours; sampling through prebuilt records and through records
+    built inside the call gives identical results."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, torch.float32)
+    rec = dm.records.cpu().numpy()
+    m = prob.mesh
+    xyz = rec[:, :12].view(np.float64).reshape(-1, 4, 3)
+    ids = rec[:, 12:14].view(np.int32)
+    nbs = rec[:, 14:16].view(np.int32)
+    assert np.array_equal(xyz, m.pos[m.cells])
+    assert np.array_equal(ids, m.cells)
+    assert np.array_equal(nbs, m.nbrs)
+    bare = api.DeviceMesh(pos=dm.pos, cells=dm.cells, nbrs=dm.nbrs)
+    fields = ["density", "velocity_gradient"]
+    spec = api.GridSpec(prob.region, (12, 11, 10))
+    for method in (0, 2):
+        a = api.sample(dm, rho, vel, None, fields, spec, method=method, samples=8, seed=3, want_cells=True)
+        b = api.sample(bare, rho, vel, None, fields, spec, method=method, samples=8, seed=3, want_cells=True)
+        for f in fields + ["located_cell"]:
+            assert torch.equal(a[f], b[f]), (method, f)
+
+
+@pytest.mark.parametrize("samples", [8, 6])
+def test_samples_outside_hull(gpu, samples):
+    """A grid reaching past the convex hull: samples outside it add zero
+    and the sum is still divided by M, as the reference does; with
+    finite_only the divisor is the number of samples inside, so per cell
+    finite_only / default = M / k for some k in 1..M."""
+    from dtfe_amd import api
+    prob = P.uniform_problem()
+    dm, rho, vel, _ = gpu_setup(prob, torch.float64)
+    box, n = (-6.0, 16.0, -6.0, 16.0, -6.0, 16.0), (22, 22, 22)
+    spec = api.GridSpec(box, n)
+    fields = ["density", "velocity"]
+    a = api.sample(dm, rho, vel, None, fields, spec, method=2, samples=samples, seed=11)
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=fields, kind="grid", box=box, n=n,
+                   method=2, samples=samples, seed=11)
+    for f in fields:
+        assert rel_err(a[f].cpu().numpy(), ref[f]) < TOL64, f
+    b = api.sample(dm, rho, vel, None, ["density"], spec, method=2, samples=samples, seed=11, finite_only=True)
+    da, db = a["density"].cpu().numpy(), b["density"].cpu().numpy()
+    nz = da > 1e-9 * da.max()
+    k = samples * da[nz] / db[nz]
+    assert np.all(np.abs(k - np.round(k)) < 1e-6)
+    assert np.all((np.round(k) >= 1) & (np.round(k) <= samples))
+    assert np.any(np.round(k) < samples), "no cell straddles the hull"
+    assert np.any(np.round(k) == samples)
