@@ -56,6 +56,9 @@ constexpr int kBlock = 256;
 #define DTFE_SAMPLE_BOUNDS __launch_bounds__(kBlock)
 #endif
 constexpr int kMaxScalars = 4;
+// Monte-Carlo / equidistant samples per cell: the one-thread-per-row kernels
+// keep one row's MT words (3 per sample) in a 64 KB LDS ring
+constexpr int kMaxSamples = 4096;
 constexpr int kMaxWalkSteps = 1 << 20;
 // Workgroup size of the one-thread-per-sample Monte-Carlo kernel.
 #ifndef DTFE_LANES_BLOCK
@@ -164,52 +167,67 @@ __device__ __forceinline__ void face_orients(const d3 X[4], d3 p, double O[4], i
     cert[3] = O[3] > b ? 1 : (O[3] < -b ? -1 : 0);
 }
 
-// EXACT = false: signs the FP64 filter cannot certify end the walk with
-// t = -3 (the row is then redone by the exact kernel). EXACT = true: such
-// signs are decided exactly with the symbolic tie-break.
+// One step of the visibility walk towards p in cell t (entered from `prev`,
+// -1 at the start): loads the cell record into L.X / L.v, computes L.O and
+// crosses the face with the most negative certified O (the face whose
+// barycentric coordinate of p is smallest). EXACT = false: a sign the FP64
+// filter cannot certify makes the step kUnsure when no face is certainly
+// negative. EXACT = true: such signs are decided exactly with the symbolic
+// tie-break. In a Delaunay triangulation the walk ends whatever negative
+// face each step takes.
+enum WalkStep { kInside = 0, kMove = 1, kOutside = 2, kUnsure = 3 };
+
+template <bool EXACT>
+__device__ __forceinline__ int walk_step(const CellRecord *__restrict__ rec, d3 p, int t, int prev, Located &L,
+                                         int &next) {
+    const CellRecord r = rec[t];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L.X[k] = d3{r.x[3 * k], r.x[3 * k + 1], r.x[3 * k + 2]};
+    L.v = r.v;
+    int cert[4];
+    face_orients(L.X, p, L.O, cert);
+    const int nb[4] = {r.nb.x, r.nb.y, r.nb.z, r.nb.w};
+    int pick = -1;
+    double best = 0.0;
+    bool unsure = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (prev >= 0 && nb[i] == prev) continue;  // p is inside that face
+        int s = cert[i];
+        if (s == 0) {
+            if (EXACT) s = face_sos(i, L.X, p);
+            else unsure = true;
+        }
+        if (s < 0 && (pick < 0 || L.O[i] < best)) {
+            pick = i;
+            best = L.O[i];
+        }
+    }
+    if (pick < 0) return unsure ? kUnsure : kInside;
+    next = pick == 0 ? nb[0] : pick == 1 ? nb[1] : pick == 2 ? nb[2] : nb[3];
+    return next < 0 ? kOutside : kMove;
+}
+
+// The walk from cell t: L.t = the cell holding p, -1 outside the hull, -2 no
+// termination, -3 a sign left uncertain (EXACT = false only).
 template <bool EXACT>
 __device__ Located locate(const CellRecord *__restrict__ rec, d3 p, int t) {
     Located L;
     int prev = -1;
     for (int step = 0; step < kMaxWalkSteps; ++step) {
         L.steps = step + 1;
-        const CellRecord r = rec[t];
-        const int4 c = r.v;
-        const int4 nb4 = r.nb;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) L.X[k] = d3{r.x[3 * k], r.x[3 * k + 1], r.x[3 * k + 2]};
-        int cert[4];
-        face_orients(L.X, p, L.O, cert);
-        int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-        int pick = -1;
-        bool unsure = false;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (pick >= 0 || unsure) continue;
-            if (prev >= 0 && nb[i] == prev) continue;
-            int s = cert[i];
-            if (s == 0) {
-                if (EXACT) s = face_sos(i, L.X, p);
-                else unsure = true;
-            }
-            if (s < 0) pick = i;
-        }
-        if (unsure) {
-            L.t = -3;
-            return L;
-        }
-        if (pick < 0) {
+        int next = -1;
+        const int r = walk_step<EXACT>(rec, p, t, prev, L, next);
+        if (r == kInside) {
             L.t = t;
-            L.v = c;
             return L;
         }
-        int nn = pick == 0 ? nb[0] : pick == 1 ? nb[1] : pick == 2 ? nb[2] : nb[3];
-        if (nn < 0) {
-            L.t = -1;
+        if (r != kMove) {
+            L.t = r == kOutside ? -1 : -3;
             return L;
         }
         prev = t;
-        t = nn;
+        t = next;
     }
     L.t = -2;
     return L;
@@ -282,16 +300,51 @@ __device__ __forceinline__ F ldf(const void *p, int64_t i) {
     return reinterpret_cast<const F *>(p)[i];
 }
 
-// Adds the field values at p in the located cell to the row accumulator.
+// The vertex values of one tetrahedron that a sample reads.
+template <typename F, int NS>
+struct CellValues {
+    F rho[4];
+    F vel[4][3];
+    F sc[4][NS > 0 ? NS : 1];
+};
+
+constexpr unsigned kGradUsers = DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE |
+                                DTFE_FIELD_VELOCITY_SHEAR | DTFE_FIELD_VELOCITY_VORTICITY;
+
+// Gathers them for the requested fields (vertex ids v; any valid ids when the
+// values will not be used).
+template <typename F, bool VEL, int NS>
+__device__ __forceinline__ void load_cell_values(const VertexFieldsDev &vf, int4 v, unsigned fields,
+                                                 CellValues<F, NS> &cv) {
+    const int id[4] = {v.x, v.y, v.z, v.w};
+    if (fields & DTFE_FIELD_DENSITY)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cv.rho[k] = ldf<F>(vf.rho, id[k]);
+    if (VEL && (fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_STD | kGradUsers)))
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cv.vel[k][c] = ldf<F>(vf.vel, 3 * (int64_t)id[k] + c);
+    if (NS > 0 && (fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT)))
+#pragma unroll
+        for (int q = 0; q < (NS > 0 ? NS : 1); ++q) {
+            if (q >= vf.S) break;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cv.sc[k][q] = ldf<F>(vf.scal, (int64_t)vf.S * id[k] + q);
+        }
+}
+
+// Adds the field values at p in the located cell (vertex values cv) to the
+// row accumulator.
 // F = double: the reference's formulas evaluated in FP64.
 // F = float : FP64 inverse cast to FP32 (as the reference's float build does)
 //             and the density as the convex barycentric sum in FP32.
 template <typename F, bool VEL, int NS>
-__device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexFieldsDev &vf, unsigned fields,
-                                           bool value_bug, RowAcc<F, NS> &acc, bool want_std) {
+__device__ __forceinline__ void add_sample_values(const Located &L, d3 p, const CellValues<F, NS> &cv, int S,
+                                                  unsigned fields, bool value_bug, RowAcc<F, NS> &acc,
+                                                  bool want_std) {
     double Ai[3][3];
     bool ok = ref_inverse(L.X, Ai);
-    const int id[4] = {L.v.x, L.v.y, L.v.z, L.v.w};
     double rel_d[3] = {p.x - L.X[0].x, p.y - L.X[0].y, p.z - L.X[0].z};
     F rel[3] = {(F)rel_d[0], (F)rel_d[1], (F)rel_d[2]};
     F AiF[3][3];
@@ -307,8 +360,7 @@ __device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexF
         for (int i = 0; i < 4; ++i) lam[i] = (F)(L.O[i] * inv);
     }
     if (fields & DTFE_FIELD_DENSITY) {
-        F r0 = ldf<F>(vf.rho, id[0]), r1 = ldf<F>(vf.rho, id[1]), r2 = ldf<F>(vf.rho, id[2]),
-          r3 = ldf<F>(vf.rho, id[3]);
+        const F r0 = cv.rho[0], r1 = cv.rho[1], r2 = cv.rho[2], r3 = cv.rho[3];
         F val;
         if (sizeof(F) == 8) {
             F dd[3] = {r1 - r0, r2 - r0, r3 - r0};
@@ -321,14 +373,8 @@ __device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexF
         }
         acc.d += val;
     }
-    constexpr unsigned kGradUsers = DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE |
-                                    DTFE_FIELD_VELOCITY_SHEAR | DTFE_FIELD_VELOCITY_VORTICITY;
     if (VEL && (fields & (DTFE_FIELD_VELOCITY | DTFE_FIELD_VELOCITY_STD | kGradUsers))) {
-        F vb[4][3];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) vb[k][c] = ldf<F>(vf.vel, 3 * (int64_t)id[k] + c);
+        const F(&vb)[4][3] = cv.vel;
         F G[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -372,13 +418,10 @@ __device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexF
                 for (int i = 0; i < 3; ++i) acc.vg[c * 3 + i] += G[i][c];
     }
     if (NS > 0 && (fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT))) {
-        const int S = vf.S;
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
+        for (int q = 0; q < (NS > 0 ? NS : 1); ++q) {
             if (q >= S) break;
-            F sb[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) sb[k] = ldf<F>(vf.scal, (int64_t)S * id[k] + q);
+            const F sb[4] = {cv.sc[0][q], cv.sc[1][q], cv.sc[2][q], cv.sc[3][q]};
             F g[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -395,6 +438,14 @@ __device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexF
                 for (int i = 0; i < 3; ++i) acc.sg[q * 3 + i] += g[i];
         }
     }
+}
+
+template <typename F, bool VEL, int NS>
+__device__ __forceinline__ void add_sample(const Located &L, d3 p, const VertexFieldsDev &vf, unsigned fields,
+                                           bool value_bug, RowAcc<F, NS> &acc, bool want_std) {
+    CellValues<F, NS> cv;
+    load_cell_values<F, VEL, NS>(vf, L.v, fields, cv);
+    add_sample_values<F, VEL, NS>(L, p, cv, vf.S, fields, value_bug, acc, want_std);
 }
 
 // A sample outside the hull contributes a zero vector to the std.
@@ -2219,8 +2270,8 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
     const int method = mc->method;
     if (method != DTFE_METHOD_CENTRE && method != DTFE_METHOD_MC && method != DTFE_METHOD_EQUI)
         return fail(ctx, DTFE_ERR_ARGUMENT, "method must be 0 (centre), 2 (Monte Carlo) or 3 (equidistant)");
-    if (method != DTFE_METHOD_CENTRE && mc->samples < 1)
-        return fail(ctx, DTFE_ERR_ARGUMENT, "samples per cell must be >= 1");
+    if (method != DTFE_METHOD_CENTRE && (mc->samples < 1 || mc->samples > kMaxSamples))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "samples per cell must be in [1, " + std::to_string(kMaxSamples) + "]");
     if (method == DTFE_METHOD_EQUI && smp->kind != DTFE_SAMPLE_GRID)
         return fail(ctx, DTFE_ERR_ARGUMENT, "equidistant averaging is only available on a regular grid");
     if ((fields & DTFE_FIELD_VELOCITY_STD) && !(smp->kind == DTFE_SAMPLE_GRID && method == DTFE_METHOD_MC))
@@ -2367,7 +2418,8 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     const int method = mc->method;
     if (method != DTFE_METHOD_CENTRE && method != DTFE_METHOD_MC && method != DTFE_METHOD_EQUI)
         return fail(ctx, DTFE_ERR_ARGUMENT, "method must be 0, 2 or 3");
-    if (method != DTFE_METHOD_CENTRE && mc->samples < 1) return fail(ctx, DTFE_ERR_ARGUMENT, "samples must be >= 1");
+    if (method != DTFE_METHOD_CENTRE && (mc->samples < 1 || mc->samples > kMaxSamples))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "samples per cell must be in [1, " + std::to_string(kMaxSamples) + "]");
     const bool vel_up = up->velocity || up->velocity_gradient || up->velocity_divergence || up->velocity_shear ||
                         up->velocity_vorticity;
     if (!vf->density) return fail(ctx, DTFE_ERR_ARGUMENT, "the adjoint needs the forward vertex densities");
