@@ -24,6 +24,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/dtfe_hip.h"
@@ -48,7 +49,7 @@ constexpr int kBlock = 256;
 #endif
 // occupancy floor of the one-thread-per-sample Monte-Carlo kernel
 #ifndef DTFE_LANES_MIN_WAVES
-#define DTFE_LANES_MIN_WAVES 4
+#define DTFE_LANES_MIN_WAVES 5
 #endif
 #if DTFE_MIN_WAVES > 0
 #define DTFE_SAMPLE_BOUNDS __launch_bounds__(kBlock, DTFE_MIN_WAVES)
@@ -168,7 +169,7 @@ __device__ __forceinline__ void face_orients(const d3 X[4], d3 p, double O[4], i
 }
 
 // One step of the visibility walk towards p in cell t (entered from `prev`,
-// -1 at the start): loads the cell record into L.X / L.v, computes L.O and
+// -2 at the start, which matches no neighbour): loads the cell record into L.X / L.v, computes L.O and
 // crosses the face with the most negative certified O (the face whose
 // barycentric coordinate of p is smallest). EXACT = false: a sign the FP64
 // filter cannot certify makes the step kUnsure when no face is certainly
@@ -187,21 +188,22 @@ __device__ __forceinline__ int walk_step(const CellRecord *__restrict__ rec, d3 
     int cert[4];
     face_orients(L.X, p, L.O, cert);
     const int nb[4] = {r.nb.x, r.nb.y, r.nb.z, r.nb.w};
+    // branch-free selection (the exact tie-break aside)
     int pick = -1;
     double best = 0.0;
     bool unsure = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        if (prev >= 0 && nb[i] == prev) continue;  // p is inside that face
+        const bool live = nb[i] != prev;  // p is inside the face back to prev
         int s = cert[i];
-        if (s == 0) {
-            if (EXACT) s = face_sos(i, L.X, p);
-            else unsure = true;
+        if (EXACT) {
+            if (live && s == 0) s = face_sos(i, L.X, p);
+        } else {
+            unsure = unsure || (live && s == 0);
         }
-        if (s < 0 && (pick < 0 || L.O[i] < best)) {
-            pick = i;
-            best = L.O[i];
-        }
+        const bool take = live && s < 0 && (pick < 0 || L.O[i] < best);
+        pick = take ? i : pick;
+        best = take ? L.O[i] : best;
     }
     if (pick < 0) return unsure ? kUnsure : kInside;
     next = pick == 0 ? nb[0] : pick == 1 ? nb[1] : pick == 2 ? nb[2] : nb[3];
@@ -213,7 +215,7 @@ __device__ __forceinline__ int walk_step(const CellRecord *__restrict__ rec, d3 
 template <bool EXACT>
 __device__ Located locate(const CellRecord *__restrict__ rec, d3 p, int t) {
     Located L;
-    int prev = -1;
+    int prev = -2;
     for (int step = 0; step < kMaxWalkSteps; ++step) {
         L.steps = step + 1;
         int next = -1;
@@ -474,55 +476,47 @@ __device__ __forceinline__ void stf(void *p, int64_t i, F v) {
 }
 
 // Writes the first `count` of C components of one output row at p[base..].
-// With LPR lanes holding the same row (all with the same values), lane w
-// writes components w, w + LPR, ..., so one store covers LPR components of
-// each row and consecutive rows' components are consecutive addresses.
-template <typename F, int LPR, int C>
-__device__ __forceinline__ void store_field(void *p, int64_t base, const F (&v)[C], int w, int count = C) {
+template <typename F, int C>
+__device__ __forceinline__ void store_field(void *p, int64_t base, const F (&v)[C], int count = C) {
     F *o = reinterpret_cast<F *>(p) + base;
 #pragma unroll
-    for (int b = 0; b < C; b += LPR) {
-        F x = v[b];
-#pragma unroll
-        for (int c = 1; c < LPR && b + c < C; ++c) x = (w == c) ? v[b + c] : x;
-        if (b + w < count) o[b + w] = x;
-    }
+    for (int c = 0; c < C; ++c)
+        if (c < count) o[c] = v[c];
 }
 
 // Writes one output row; div/shear/vort follow DTFE.cpp:394-433.
-// LPR > 1: called by all LPR lanes of the row, w = lane within the row.
-template <typename F, bool VEL, int NS, int LPR = 1>
+template <typename F, bool VEL, int NS>
 __device__ __forceinline__ void store_row(const OutPtrs &o, int64_t row, const RowAcc<F, NS> &acc, F scale,
-                                          unsigned fields, int S, int w = 0) {
+                                          unsigned fields, int S) {
     if (fields & DTFE_FIELD_DENSITY) {
         const F v[1] = {acc.d * scale};
-        store_field<F, LPR>(o.density, row, v, w);
+        store_field<F>(o.density, row, v);
     }
     if (VEL) {
         if (fields & DTFE_FIELD_VELOCITY) {
             const F v[3] = {acc.v[0] * scale, acc.v[1] * scale, acc.v[2] * scale};
-            store_field<F, LPR>(o.velocity, 3 * row, v, w);
+            store_field<F>(o.velocity, 3 * row, v);
         }
         if (fields & (DTFE_FIELD_VELOCITY_GRADIENT | DTFE_FIELD_VELOCITY_DIVERGENCE | DTFE_FIELD_VELOCITY_SHEAR |
                       DTFE_FIELD_VELOCITY_VORTICITY)) {
             F g[9];
 #pragma unroll
             for (int i = 0; i < 9; ++i) g[i] = acc.vg[i] * scale;
-            if (fields & DTFE_FIELD_VELOCITY_GRADIENT) store_field<F, LPR>(o.velgrad, 9 * row, g, w);
+            if (fields & DTFE_FIELD_VELOCITY_GRADIENT) store_field<F>(o.velgrad, 9 * row, g);
             if (fields & DTFE_FIELD_VELOCITY_DIVERGENCE) {
                 const F v[1] = {g[0] + g[4] + g[8]};
-                store_field<F, LPR>(o.div, row, v, w);
+                store_field<F>(o.div, row, v);
             }
             if (fields & DTFE_FIELD_VELOCITY_SHEAR) {
                 F t0 = (g[0] + g[0]) / (F)2, t1 = (g[1] + g[3]) / (F)2, t2 = (g[2] + g[6]) / (F)2;
                 F t3 = (g[4] + g[4]) / (F)2, t4 = (g[5] + g[7]) / (F)2;
                 F tr = (t0 + t3 + g[8]) / (F)3;
                 const F v[5] = {t0 - tr, t1, t2, t3 - tr, t4};
-                store_field<F, LPR>(o.shear, 5 * row, v, w);
+                store_field<F>(o.shear, 5 * row, v);
             }
             if (fields & DTFE_FIELD_VELOCITY_VORTICITY) {
                 const F v[3] = {(g[1] - g[3]) / (F)2, (g[2] - g[6]) / (F)2, (g[5] - g[7]) / (F)2};
-                store_field<F, LPR>(o.vort, 3 * row, v, w);
+                store_field<F>(o.vort, 3 * row, v);
             }
         }
     }
@@ -532,13 +526,13 @@ __device__ __forceinline__ void store_row(const OutPtrs &o, int64_t row, const R
             F v[C];
 #pragma unroll
             for (int q = 0; q < C; ++q) v[q] = acc.s[q] * scale;
-            store_field<F, LPR>(o.scalar, (int64_t)S * row, v, w, S);
+            store_field<F>(o.scalar, (int64_t)S * row, v, S);
         }
         if (fields & DTFE_FIELD_SCALAR_GRADIENT) {
             F v[3 * C];
 #pragma unroll
             for (int q = 0; q < 3 * C; ++q) v[q] = acc.sg[q] * scale;
-            store_field<F, LPR>(o.scalargrad, (int64_t)3 * S * row, v, w, 3 * S);
+            store_field<F>(o.scalargrad, (int64_t)3 * S * row, v, 3 * S);
         }
     }
 }
@@ -1077,10 +1071,34 @@ __global__ void __launch_bounds__(kBlock) k_locate_centres(SampleArgs a, int32_t
     }
 }
 
+// Sums over aligned groups of LPR (2, 4, 8 or 16) lanes, the total in every
+// lane of the group, with DPP lane moves (no LDS round trip): quad
+// permutations [1,0,3,2] and [2,3,0,1], then mirrors within 8 and 16 lanes
+// (the partial sums are uniform within each quad / half by then).
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_move_t(T v) {
+    if constexpr (sizeof(T) == 8) {
+        const int lo = dpp_move<CTRL>(__double2loint(v)), hi = dpp_move<CTRL>(__double2hiint(v));
+        return __hiloint2double(hi, lo);
+    } else if constexpr (std::is_same<T, float>::value) {
+        return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+    } else {
+        return dpp_move<CTRL>(v);
+    }
+}
+
 template <int LPR, typename T>
 __device__ __forceinline__ T lane_sum(T v) {
-#pragma unroll
-    for (int o = 1; o < LPR; o <<= 1) v += __shfl_xor(v, o, 64);
+    static_assert(LPR == 2 || LPR == 4 || LPR == 8 || LPR == 16, "lane groups of 2, 4, 8 or 16");
+    v += dpp_move_t<0xB1>(v);                      // quad_perm [1,0,3,2]
+    if constexpr (LPR >= 4) v += dpp_move_t<0x4E>(v);   // quad_perm [2,3,0,1]
+    if constexpr (LPR >= 8) v += dpp_move_t<0x141>(v);  // row_half_mirror
+    if constexpr (LPR >= 16) v += dpp_move_t<0x140>(v); // row_mirror
     return v;
 }
 
@@ -1188,7 +1206,9 @@ __global__ void __launch_bounds__(kLanesBlock, DTFE_LANES_MIN_WAVES) k_sample_mc
             F scale;
             if (a.flags & DTFE_AVERAGE_FINITE_ONLY) scale = nfin > 0 ? (F)1 / (F)nfin : (F)0;
             else scale = (F)1 / (F)M;
-            store_row<F, VEL, NS, LPR>(a.out, row, acc, scale, a.fields, a.vf.S, w);
+            // one lane per row writes it (spreading the components over the
+            // row's lanes needs a lane-indexed select that spills to scratch)
+            if (w == 0) store_row<F, VEL, NS>(a.out, row, acc, scale, a.fields, a.vf.S);
             if (want_std && w == 0) stf<F>(a.out.velstd, row, sd);
         }
     }
