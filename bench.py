@@ -129,6 +129,15 @@ def main():
     w = torch.from_numpy(M.vertex_data(m, prep.weight)).to(dev, dt)
     vel = torch.from_numpy(M.vertex_data(m, prep.vel)).to(dev, dt)
     dm = api.DeviceMesh.from_mesh(m, dev)
+    # the point-location index is built once per mesh (like the tessellation);
+    # time one rebuild of it for the record
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    dm.build_locator()
+    e1.record()
+    torch.cuda.synchronize()
+    locator_ms = e0.elapsed_time(e1)
     box = prep.region
     n = (args.grid,) * 3
     spec = api.GridSpec(box, n)
@@ -225,7 +234,8 @@ def main():
             "tessellation_s": round(m.seconds, 3),
         },
         "kernels_ms": {"sample": k_sample_ms, "locate_centres": k_centre_ms, "exact": k_exact_ms,
-                       "seed_grid": k_seed_ms, "vertex_density": k_vd_ms},
+                       "locator_in_step": k_seed_ms, "vertex_density": k_vd_ms},
+        "locator_build_ms_once_per_mesh": locator_ms,
         "deferred_rows": deferred,
         "walk_steps": {"centre_per_row": walk[0] / max(1, walk[2]), "per_sample": walk[1] / max(1, walk[3])}
         if walk[0] else None,

@@ -49,7 +49,7 @@ EXPORTED = [
     "dtfe_hip_create", "dtfe_hip_destroy", "dtfe_hip_set_stream", "dtfe_hip_synchronize",
     "dtfe_hip_last_error", "dtfe_hip_vertex_density", "dtfe_hip_sample", "dtfe_hip_deposit",
     "dtfe_hip_version", "dtfe_hip_last_timings", "dtfe_hip_last_deferred", "dtfe_hip_walk_failures",
-    "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint", "dtfe_hip_cell_records",
+    "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint", "dtfe_hip_locator_bytes", "dtfe_hip_build_locator",
 ]
 
 CELL_RECORD_BYTES = 128
@@ -57,7 +57,7 @@ CELL_RECORD_BYTES = 128
 
 class Mesh(C.Structure):
     _fields_ = [("n_vertices", C.c_int64), ("n_cells", C.c_int64), ("pos", C.c_void_p),
-                ("cells", C.c_void_p), ("neighbors", C.c_void_p), ("cell_records", C.c_void_p)]
+                ("cells", C.c_void_p), ("neighbors", C.c_void_p), ("locator", C.c_void_p)]
 
 
 class VertexFields(C.Structure):
@@ -124,7 +124,9 @@ def load(path: str = LIB_PATH):
         lib.dtfe_hip_last_error.restype = C.c_char_p
         lib.dtfe_hip_version.restype = C.c_char_p
         lib.dtfe_hip_vertex_density.argtypes = [vp, C.POINTER(Mesh), i32, vp, C.c_double, vp]
-        lib.dtfe_hip_cell_records.argtypes = [vp, C.POINTER(Mesh), vp]
+        lib.dtfe_hip_locator_bytes.argtypes = [i64, i64]
+        lib.dtfe_hip_locator_bytes.restype = C.c_size_t
+        lib.dtfe_hip_build_locator.argtypes = [vp, C.POINTER(Mesh), vp, C.c_size_t]
         lib.dtfe_hip_sample.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), C.c_uint,
                                         C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Outputs)]
         lib.dtfe_hip_deposit.argtypes = [vp, i32, i64, vp, i32, vp, vp, C.POINTER(C.c_double * 6),
@@ -137,7 +139,7 @@ def load(path: str = LIB_PATH):
                                          C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Upstream),
                                          C.POINTER(VertexGrads)]
         for name in EXPORTED:
-            if name not in ("dtfe_hip_last_error", "dtfe_hip_version"):
+            if name not in ("dtfe_hip_last_error", "dtfe_hip_version", "dtfe_hip_locator_bytes"):
                 getattr(lib, name).restype = C.c_int
         _lib = lib
         return lib

@@ -67,45 +67,47 @@ class DeviceMesh:
     pos: torch.Tensor    # [nv, 3] float64
     cells: torch.Tensor  # [nc, 4] int32
     nbrs: torch.Tensor   # [nc, 4] int32
-    # [nc, 16] int64: the 128-byte walk record of every cell (positions copied
-    # from pos); None: each sampling call builds them itself
-    records: Optional[torch.Tensor] = None
+    # point-location index (uint8 device bytes: one 128-byte record per cell,
+    # positions copied from pos, then a seed grid); None: each sampling call
+    # builds one itself
+    locator: Optional[torch.Tensor] = None
 
     @staticmethod
-    def from_mesh(mesh: Mesh, device="cuda", records: bool = True) -> "DeviceMesh":
+    def from_mesh(mesh: Mesh, device="cuda", locator: bool = True) -> "DeviceMesh":
         dev = torch.device(device)
         dm = DeviceMesh(pos=torch.from_numpy(np.ascontiguousarray(mesh.pos)).to(dev),
                         cells=torch.from_numpy(np.ascontiguousarray(mesh.cells)).to(dev),
                         nbrs=torch.from_numpy(np.ascontiguousarray(mesh.nbrs)).to(dev))
-        if records:
-            dm.build_records()
+        if locator and dm.pos.shape[0] >= 4 and dm.cells.shape[0] > 0:
+            dm.build_locator()
         return dm
 
     @property
     def device(self):
         return self.pos.device
 
-    def build_records(self) -> None:
-        """(Re)builds the cell records from pos/cells/nbrs; call it again
-        after changing pos in place."""
+    def build_locator(self) -> None:
+        """(Re)builds the locator from pos/cells/nbrs; call it again after
+        changing pos in place."""
         ctx = context(self.device)
-        self.records = None
-        rec = torch.empty((self.cells.shape[0], N.CELL_RECORD_BYTES // 8), dtype=torch.int64,
-                          device=self.device)
+        self.locator = None
+        nbytes = int(ctx.lib.dtfe_hip_locator_bytes(self.pos.shape[0], self.cells.shape[0]))
+        # torch's caching allocator hands out 512-byte aligned blocks
+        loc = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         cm = self.c()
-        ctx.check(ctx.lib.dtfe_hip_cell_records(ctx.h, C.byref(cm), _ptr(rec)))
-        self.records = rec
+        ctx.check(ctx.lib.dtfe_hip_build_locator(ctx.h, C.byref(cm), _ptr(loc), nbytes))
+        self.locator = loc
 
     def c(self) -> N.Mesh:
         for t in (self.pos, self.cells, self.nbrs):
             assert t.is_contiguous() and t.is_cuda
         assert self.pos.dtype == torch.float64 and self.cells.dtype == torch.int32
-        rec = None
-        if self.records is not None:
-            assert self.records.is_contiguous() and self.records.shape[0] == self.cells.shape[0]
-            rec = self.records.data_ptr()
+        loc = None
+        if self.locator is not None:
+            assert self.locator.is_contiguous() and self.locator.data_ptr() % 256 == 0
+            loc = self.locator.data_ptr()
         return N.Mesh(self.pos.shape[0], self.cells.shape[0], self.pos.data_ptr(),
-                      self.cells.data_ptr(), self.nbrs.data_ptr(), rec)
+                      self.cells.data_ptr(), self.nbrs.data_ptr(), loc)
 
 
 def vertex_density(mesh: DeviceMesh, weight: torch.Tensor, rho_bar: float) -> torch.Tensor:

@@ -360,11 +360,12 @@ void dtfe_interpolation(std::vector<Particle_data> *p, std::vector<Sample_point>
            d_rho(std::max<size_t>(1, w.size()) * sizeof(Real));
     dtfe_mesh mesh{nv, (int64_t)cells.size() / 4, d_pos.as<double>(), d_cells.as<int32_t>(), d_nbrs.as<int32_t>(),
                    nullptr};
-    // the walk records of the cells, built once for every field sampled below
-    DevMem d_rec((size_t)mesh.n_cells * DTFE_CELL_RECORD_BYTES);
+    // the point-location index, built once for every field sampled below
+    const size_t loc_bytes = dtfe_hip_locator_bytes(nv, mesh.n_cells);
+    DevMem d_loc(loc_bytes);
     if (mesh.n_cells > 0 && nv >= 4) {
-        ctx.check(dtfe_hip_cell_records(ctx.h, &mesh, d_rec.p), "cell records");
-        mesh.cell_records = d_rec.p;
+        ctx.check(dtfe_hip_build_locator(ctx.h, &mesh, d_loc.p, loc_bytes), "point-location index");
+        mesh.locator = d_loc.p;
     }
     if (nv < 4)
         dtfe_dropin::warn(o, "fewer than 4 vertices in the Delaunay triangulation: all values are set to 0");

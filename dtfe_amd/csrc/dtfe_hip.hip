@@ -45,7 +45,7 @@ constexpr int kBlock = 256;
 // Seed-grid buckets per mesh vertex (walks start from the seed of the bucket
 // holding the point).
 #ifndef DTFE_SEED_BUCKETS_PER_VERTEX
-#define DTFE_SEED_BUCKETS_PER_VERTEX 0.5
+#define DTFE_SEED_BUCKETS_PER_VERTEX 2.0
 #endif
 // occupancy floor of the one-thread-per-sample Monte-Carlo kernel
 #ifndef DTFE_LANES_MIN_WAVES
@@ -1688,7 +1688,7 @@ struct dtfe_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf W, hull, seed_a, seed_b, bbox_part, bbox, status, axes, trig, equi, plan, flags, rows, mass, mom,
+    DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, plan, flags, rows, mass, mom,
         defer_rows, defer_words, adj_rho, adj_dW, adj_partial, centre, records;
     int last_deferred = 0;
     int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples
@@ -1759,51 +1759,85 @@ int launch_cell_records(dtfe_ctx *ctx, const dtfe_mesh *m, void *records) {
     return DTFE_OK;
 }
 
-// The walk records: the caller's, or built for this call.
-int use_cell_records(dtfe_ctx *ctx, const dtfe_mesh *m, SampleArgs &a) {
-    if (m->cell_records) {
-        if ((uintptr_t)m->cell_records % 16)
-            return fail(ctx, DTFE_ERR_ARGUMENT, "cell_records must be 16-byte aligned");
-        a.rec = (const CellRecord *)m->cell_records;
-        return DTFE_OK;
-    }
-    int rc;
-    if ((rc = ensure(ctx, ctx->records, (size_t)m->n_cells * sizeof(CellRecord)))) return rc;
-    if ((rc = launch_cell_records(ctx, m, ctx->records.p))) return rc;
-    a.rec = (const CellRecord *)ctx->records.p;
-    return DTFE_OK;
-}
-
-// Builds the seed grid of the mesh; returns it in `sg`.
-int build_seed_grid(dtfe_ctx *ctx, const dtfe_mesh *m, SeedGrid &sg) {
-    int64_t nv = m->n_vertices, nc = m->n_cells;
+// Seed-grid buckets per axis for a mesh of nv vertices.
+int seed_grid_size(int64_t nv) {
     int g = (int)std::max<double>(
         1.0, std::floor(std::cbrt(std::max<double>(1.0, nv * (double)DTFE_SEED_BUCKETS_PER_VERTEX))));
-    g = std::min(g, 512);
+    return std::min(g, 512);
+}
+
+// The locator of a mesh in one buffer: cell records, then (256-byte aligned)
+// the seed grid's box (lo[3], 1/h[3]) and its g^3 cells.
+struct LocatorLayout {
+    size_t bbox, seeds, total;
+    int g;
+};
+
+LocatorLayout locator_layout(int64_t nv, int64_t nc) {
+    LocatorLayout L;
+    L.g = seed_grid_size(nv);
+    L.bbox = ((size_t)nc * sizeof(CellRecord) + 255) & ~(size_t)255;
+    L.seeds = L.bbox + 256;
+    L.total = L.seeds + (size_t)L.g * L.g * L.g * sizeof(int);
+    return L;
+}
+
+// Builds the seed grid of the mesh into bbox_out (6 doubles) and seed_out
+// (g^3 cells): every bucket holds a tetrahedron near it.
+int build_seed_grid(dtfe_ctx *ctx, const dtfe_mesh *m, int g, double *bbox_out, int *seed_out) {
+    int64_t nv = m->n_vertices, nc = m->n_cells;
     int64_t nb = (int64_t)g * g * g;
     int parts = std::min(1024, grid_for(nv));
     int rc;
     if ((rc = ensure(ctx, ctx->bbox_part, 6 * sizeof(double) * parts))) return rc;
-    if ((rc = ensure(ctx, ctx->bbox, 6 * sizeof(double)))) return rc;
-    if ((rc = ensure(ctx, ctx->seed_a, nb * sizeof(int)))) return rc;
     if ((rc = ensure(ctx, ctx->seed_b, nb * sizeof(int)))) return rc;
     hipLaunchKernelGGL(k_bbox_partial, dim3(parts), dim3(kBlock), 0, ctx->stream, m->pos, nv,
                        (double *)ctx->bbox_part.p);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, ctx->stream, (const double *)ctx->bbox_part.p, parts, g,
-                       g, g, (double *)ctx->bbox.p);
-    hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, (int *)ctx->seed_a.p, nb, -1);
-    sg.seed = (const int *)ctx->seed_a.p;
-    sg.bbox = (const double *)ctx->bbox.p;
+                       g, g, bbox_out);
+    hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, seed_out, nb, -1);
+    SeedGrid sg;
+    sg.seed = seed_out;
+    sg.bbox = bbox_out;
     sg.g[0] = sg.g[1] = sg.g[2] = g;
     hipLaunchKernelGGL(k_seed_cells, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
-                       (const int4 *)m->cells, nc, sg, (int *)ctx->seed_a.p);
-    int *in = (int *)ctx->seed_a.p, *out = (int *)ctx->seed_b.p;
+                       (const int4 *)m->cells, nc, sg, seed_out);
+    // empty buckets take a neighbour's seed; an even number of passes ends in seed_out
+    int *in = seed_out, *out = (int *)ctx->seed_b.p;
     for (int it = 0; it < 8; ++it) {
         hipLaunchKernelGGL(k_seed_fill, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, in, out, g, g, g);
         std::swap(in, out);
     }
-    sg.seed = in;
     HIP_TRY(hipGetLastError());
+    return DTFE_OK;
+}
+
+int build_locator(dtfe_ctx *ctx, const dtfe_mesh *m, char *loc) {
+    const LocatorLayout L = locator_layout(m->n_vertices, m->n_cells);
+    int rc;
+    if ((rc = launch_cell_records(ctx, m, loc))) return rc;
+    return build_seed_grid(ctx, m, L.g, (double *)(loc + L.bbox), (int *)(loc + L.seeds));
+}
+
+// The walk records and seed grid: the caller's locator, or one built for
+// this call in the context's scratch buffer.
+int use_locator(dtfe_ctx *ctx, const dtfe_mesh *m, SampleArgs &a) {
+    const LocatorLayout L = locator_layout(m->n_vertices, m->n_cells);
+    const char *loc = (const char *)m->locator;
+    int rc;
+    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
+    if (!loc) {
+        if ((rc = ensure(ctx, ctx->records, L.total))) return rc;
+        if ((rc = build_locator(ctx, m, (char *)ctx->records.p))) return rc;
+        loc = (const char *)ctx->records.p;
+    } else if ((uintptr_t)loc % 256) {
+        return fail(ctx, DTFE_ERR_ARGUMENT, "locator must be 256-byte aligned");
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
+    a.rec = (const CellRecord *)loc;
+    a.sg.bbox = (const double *)(loc + L.bbox);
+    a.sg.seed = (const int *)(loc + L.seeds);
+    a.sg.g[0] = a.sg.g[1] = a.sg.g[2] = L.g;
     return DTFE_OK;
 }
 
@@ -2007,9 +2041,6 @@ int setup_tables(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_mc *mc, SampleArg
                                ctx->stream));
         a.equi = (const double *)ctx->equi.p;
     }
-    HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
-    if ((rc = build_seed_grid(ctx, m, a.sg))) return rc;
-    HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream));
     if ((rc = ensure(ctx, ctx->status, 64))) return rc;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->status.p, (int64_t)16, 0);
     a.status = (int *)ctx->status.p;
@@ -2176,7 +2207,7 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     if (!ctx) return DTFE_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_a, &ctx->seed_b, &ctx->bbox_part, &ctx->bbox,
+    DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_b, &ctx->bbox_part,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
                       &ctx->adj_partial, &ctx->centre, &ctx->records};
@@ -2203,16 +2234,24 @@ int dtfe_hip_synchronize(dtfe_ctx *ctx) {
 
 const char *dtfe_hip_last_error(dtfe_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int dtfe_hip_cell_records(dtfe_ctx *ctx, const dtfe_mesh *m, void *records) {
+size_t dtfe_hip_locator_bytes(int64_t n_vertices, int64_t n_cells) {
+    if (n_vertices < 0 || n_cells < 0) return 0;
+    return locator_layout(n_vertices, n_cells).total;
+}
+
+int dtfe_hip_build_locator(dtfe_ctx *ctx, const dtfe_mesh *m, void *locator, size_t bytes) {
     if (!ctx) return DTFE_ERR_ARGUMENT;
     int rc = check_mesh(ctx, m);
     if (rc) return rc;
-    if (m->n_cells > 0 && (!m->pos || !m->cells || !m->neighbors))
-        return fail(ctx, DTFE_ERR_ARGUMENT, "mesh arrays are NULL");
-    if (m->n_cells > 0 && !records) return fail(ctx, DTFE_ERR_ARGUMENT, "records is NULL");
-    if ((uintptr_t)records % 16) return fail(ctx, DTFE_ERR_ARGUMENT, "records must be 16-byte aligned");
+    if (m->n_vertices < 4 || m->n_cells == 0)
+        return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: nothing to locate in");
+    if (!m->pos || !m->cells || !m->neighbors) return fail(ctx, DTFE_ERR_ARGUMENT, "mesh arrays are NULL");
+    if (!locator) return fail(ctx, DTFE_ERR_ARGUMENT, "locator is NULL");
+    if ((uintptr_t)locator % 256) return fail(ctx, DTFE_ERR_ARGUMENT, "locator must be 256-byte aligned");
+    if (bytes < locator_layout(m->n_vertices, m->n_cells).total)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "locator buffer smaller than dtfe_hip_locator_bytes()");
     HIP_TRY(hipSetDevice(ctx->device));
-    return launch_cell_records(ctx, m, records);
+    return build_locator(ctx, m, (char *)locator);
 }
 
 int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *m, int dtype, const void *weight, double rho_bar,
@@ -2345,7 +2384,7 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
                                out->located_cell, n_rows, -1);
         return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: fields are zero");
     }
-    if ((rc = use_cell_records(ctx, m, a))) return rc;
+    if ((rc = use_locator(ctx, m, a))) return rc;
     if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
     const int ns = want_scal ? vf->n_scalar : 0;
     return launch_rows(ctx, a, mc, vf->dtype == DTFE_F64, want_vel, ns, false);
@@ -2486,7 +2525,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     std::vector<double> axes, trig;
     if ((rc = setup_rows(ctx, smp, method, a, axes, trig))) return rc;
     if (a.n_rows > 0) {
-        if ((rc = use_cell_records(ctx, m, a))) return rc;
+        if ((rc = use_locator(ctx, m, a))) return rc;
         if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
         if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true))) return rc;
     }

@@ -79,14 +79,17 @@ typedef struct {
     const double  *pos;        /* [n_vertices][3] */
     const int32_t *cells;      /* [n_cells][4] */
     const int32_t *neighbors;  /* [n_cells][4] */
-    /* Optional: [n_cells] records of DTFE_CELL_RECORD_BYTES filled by
-     * dtfe_hip_cell_records from this pos/cells/neighbors (the point-location
-     * walk reads one record per tetrahedron). NULL: each sampling call builds
-     * them itself. They copy the positions: rebuild after changing pos. */
-    const void *cell_records;
+    /* Optional point-location index of this mesh, dtfe_hip_locator_bytes()
+     * bytes of device memory filled by dtfe_hip_build_locator(). NULL: each
+     * sampling call builds one for itself. It copies the positions: rebuild
+     * it after changing pos. */
+    const void *locator;
 } dtfe_mesh;
 
-/* Bytes per cell record: 4 FP64 vertex positions, 4 vertex ids, 4 neighbours. */
+/* The locator starts with one record per tetrahedron of
+ * DTFE_CELL_RECORD_BYTES: 4 FP64 vertex positions, 4 vertex ids, 4
+ * neighbours (a walk step reads one record); a seed grid of tetrahedra
+ * follows. */
 #define DTFE_CELL_RECORD_BYTES 128
 
 typedef struct {
@@ -139,12 +142,16 @@ int dtfe_hip_set_stream(dtfe_ctx *ctx, void *hip_stream);
 int dtfe_hip_synchronize(dtfe_ctx *ctx);
 const char *dtfe_hip_last_error(dtfe_ctx *ctx);
 
-/* Fills `records` (device memory, n_cells * DTFE_CELL_RECORD_BYTES, 16-byte
- * aligned) from the mesh's pos/cells/neighbors: the device-resident form of
- * the triangulation's cells, each with its vertex points, as the reference's
- * CGAL cells reach them (src/CGAL_triangulation/triangulation.cpp:162-195).
- * Build once per mesh and pass it as dtfe_mesh.cell_records. */
-int dtfe_hip_cell_records(dtfe_ctx *ctx, const dtfe_mesh *mesh, void *records);
+/* Bytes of the locator of a mesh with these sizes. */
+size_t dtfe_hip_locator_bytes(int64_t n_vertices, int64_t n_cells);
+/* Builds the locator (device memory of at least dtfe_hip_locator_bytes(),
+ * 256-byte aligned) from the mesh's pos/cells/neighbors: the device-resident
+ * form of the triangulation's cells with their vertex points, and the
+ * starting cells of point location, which the reference gets from CGAL's
+ * triangulation and its locate hint (src/CGAL_triangulation/triangulation.cpp:162-195,
+ * src/CGAL_triangulation/unaveraged_interpolation.cc:289). Build it once per
+ * mesh and pass it as dtfe_mesh.locator. */
+int dtfe_hip_build_locator(dtfe_ctx *ctx, const dtfe_mesh *mesh, void *locator, size_t bytes);
 
 /* Vertex density, replacing vertexDensity() (src/CGAL_triangulation/triangulation.cpp:200-279):
  * W_v = sum of the volumes of the finite tetrahedra around v,
@@ -206,9 +213,9 @@ const char *dtfe_hip_version(void);
 /* ---- diagnostics (no counterpart in the reference) ------------------------------ */
 /* Kernel times of the last calls, from HIP events on the context's stream
  * (synchronises it): ms[0] main sampling kernel (the per-sample kernel of
- * the Monte-Carlo lanes path), ms[1] exact-sign kernel, ms[2] seed-grid
- * build, ms[3] vertex density, ms[4] cell-centre location of the lanes path
- * (0 otherwise). */
+ * the Monte-Carlo lanes path), ms[1] exact-sign kernel, ms[2] locator build
+ * inside the call (0 when the mesh brings its own), ms[3] vertex density,
+ * ms[4] cell-centre location of the lanes path (0 otherwise). */
 int dtfe_hip_last_timings(dtfe_ctx *ctx, double ms[5]);
 /* Rows of the last dtfe_hip_sample call that needed exact signs. */
 int dtfe_hip_last_deferred(dtfe_ctx *ctx);

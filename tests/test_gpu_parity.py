@@ -290,14 +290,15 @@ def test_derived_fields_without_gradient(gpu, method):
         assert e < TOL64, f
 
 
-def test_cell_records_layout_and_use(gpu):
-    """Cell records hold each tetrahedron's four FP64 positions, vertex ids
-    and neighbours; sampling through prebuilt records and through records
-    built inside the call gives identical results."""
+def test_locator_layout_and_use(gpu):
+    """The locator's cell records hold each tetrahedron's four FP64
+    positions, vertex ids and neighbours; sampling through a prebuilt
+    locator and through one built inside the call gives identical results."""
     from dtfe_amd import api
     prob = P.clustered_problem()
     dm, rho, vel, sc = gpu_setup(prob, torch.float32)
-    rec = dm.records.cpu().numpy()
+    nc = prob.mesh.n_cells
+    rec = dm.locator[: nc * 128].cpu().numpy().view(np.int64).reshape(nc, 16)
     m = prob.mesh
     xyz = rec[:, :12].view(np.float64).reshape(-1, 4, 3)
     ids = rec[:, 12:14].view(np.int32)
