@@ -41,8 +41,10 @@ def parse():
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--tess", default="auto")
     ap.add_argument("--mesh-cache", default="")
-    ap.add_argument("--cpu-slabs", type=int, default=256,
-                    help="x-slabs of the grid timed on the CPU oracle (256 = the whole 256^3 grid, ~15 s)")
+    ap.add_argument("--cpu-slabs", type=int, default=32,
+                    help="x-slabs of the grid timed on one CPU thread")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="threads of the multi-threaded CPU oracle run over the whole grid")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
     ap.add_argument("--seed", type=int, default=2)
@@ -251,7 +253,11 @@ def main():
 
 
 def cpu_baseline(args, prep, m, box, n, mc_seed):
-    """The CPU oracle (FP64, one thread) on the first x-slabs of the grid."""
+    """The CPU oracle (the reference's algorithm restated in C, FP64): the
+    whole grid split into x-slab partitions over worker threads, each with the
+    reference's per-partition seed (seed + 1000 partNo, as its OpenMP run),
+    and one x-slab range on a single thread."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     from dtfe_amd import mesh as M
     w = M.vertex_data(m, prep.weight)
@@ -259,20 +265,37 @@ def cpu_baseline(args, prep, m, box, n, mc_seed):
     t0 = time.perf_counter()
     rho, _, _ = O.vertex_density(m, w, prep.rho_bar)
     t_vd = time.perf_counter() - t0
+    fields = ("density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity")
     dx = (box[1] - box[0]) / n[0]
-    slabs = max(1, args.cpu_slabs)
-    sub = (box[0], box[0] + slabs * dx, box[2], box[3], box[4], box[5])
+
+    def run(first, count, part_no):
+        sub = (box[0] + first * dx, box[0] + (first + count) * dx, box[2], box[3], box[4], box[5])
+        O.sample(m, rho, vel, None, fields=fields, kind="grid", box=sub, n=(count, n[1], n[2]),
+                 method=2, samples=args.samples, seed=mc_seed, part_no=part_no)
+        return count * n[1] * n[2] * args.samples
+
+    # one thread (the ctypes call releases the GIL, so threads run in parallel)
+    slabs1 = max(1, min(n[0], args.cpu_slabs))
     t0 = time.perf_counter()
-    O.sample(m, rho, vel, None, fields=("density", "velocity", "velocity_gradient", "velocity_divergence",
-                                        "velocity_vorticity"), kind="grid", box=sub, n=(slabs, n[1], n[2]),
-             method=2, samples=args.samples, seed=mc_seed)
-    t_s = time.perf_counter() - t0
-    ns = slabs * n[1] * n[2] * args.samples
-    return {"value": ns / t_s, "unit": "samples/s", "cores": 1, "kind": "port",
-            "sample": f"{slabs} x-slab(s) of the {n[0]}^3 grid = {ns} MC samples, FP64 oracle "
-                      f"(per-sample walk + inverse, as the reference); vertex density of all "
-                      f"{m.n_vertices} vertices took {t_vd:.2f} s separately",
-            "seconds": t_s, "host_cpu": platform.processor() or platform.machine(),
+    ns1 = run(0, slabs1, -1)
+    t1 = time.perf_counter() - t0
+    workers = max(1, min(args.cpu_workers, os.cpu_count() or 1, n[0]))
+    per = [n[0] // workers + (1 if k < n[0] % workers else 0) for k in range(workers)]
+    starts = np.concatenate([[0], np.cumsum(per)[:-1]]).tolist()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        nsw = sum(ex.map(lambda k: run(starts[k], per[k], k), range(workers)))
+    tw = time.perf_counter() - t0
+    return {"value": nsw / tw, "unit": "samples/s", "cores": workers, "kind": "port",
+            "sample": f"the whole {n[0]}x{n[1]}x{n[2]} grid x {args.samples} MC samples ({nsw} samples) split "
+                      f"into {workers} x-slab partitions on {workers} threads, FP64 oracle (per-sample walk + "
+                      f"inverse, as the reference; partition k draws from seed + 1000 k as the reference's "
+                      f"OpenMP run does)",
+            "seconds": tw,
+            "single_thread": {"value": ns1 / t1, "samples": ns1, "seconds": t1,
+                              "sample": f"{slabs1} x-slab(s), one partition"},
+            "vertex_density_seconds_one_thread": t_vd,
+            "host_cpu": platform.processor() or platform.machine(),
             "host_threads_available": os.cpu_count()}
 
 

@@ -36,6 +36,7 @@
 namespace dtfe {
 
 constexpr int kBlock = 256;
+constexpr int kXCDs = 8;  // MI355X: workgroups go round-robin to 8 dies, each with its own L2
 
 // Occupancy floor of the sampling kernels (waves per SIMD); 0 lets the
 // compiler choose the register budget.
@@ -1052,10 +1053,55 @@ __device__ __forceinline__ void mt_generate_block(uint32_t *ring, int64_t q, int
 // ----------------------------------------------------------------------------
 // centre[row]: located tetrahedron, -1 outside the hull, -3 uncertified sign
 // (the row then goes to the exact kernel from pass B, which holds its words).
+// Grids and cones go brick by brick (4 x 8 x 8 cells per workgroup, 4^3
+// bricks per tile, the tiles of each XCD consecutive), so the workgroups
+// resident on one XCD walk a compact part of the mesh through its L2.
+#ifndef DTFE_CENTRE_BRICKS
+#define DTFE_CENTRE_BRICKS 1
+#endif
+constexpr int kBrick[3] = {4, 8, 8};
+constexpr int kTile = 4;  // bricks per tile edge
+
+struct BrickGrid {
+    int nb[3];  // bricks per axis
+    int nt[3];  // tiles per axis
+    int64_t blocks() const { return (int64_t)nt[0] * nt[1] * nt[2] * kTile * kTile * kTile; }
+};
+
+BrickGrid brick_grid(const int64_t n[3]) {
+    BrickGrid g;
+    for (int d = 0; d < 3; ++d) {
+        g.nb[d] = (int)((n[d] + kBrick[d] - 1) / kBrick[d]);
+        g.nt[d] = (g.nb[d] + kTile - 1) / kTile;
+    }
+    return g;
+}
+
+// The row of this thread in brick order, -1 past the grid's edge.
+__device__ __forceinline__ int64_t brick_row(const SampleArgs &a, const BrickGrid &g) {
+    const int64_t nblk = gridDim.x;
+    const int64_t i = blockIdx.x, q = nblk / kXCDs, r = nblk % kXCDs, x = i % kXCDs;
+    const int64_t lb = x * q + min(x, r) + i / kXCDs;  // XCD x runs one consecutive range
+    const int64_t tile = lb / (kTile * kTile * kTile);
+    const int w = (int)(lb % (kTile * kTile * kTile));
+    const int64_t tk = tile % g.nt[2], tj = (tile / g.nt[2]) % g.nt[1], ti = tile / ((int64_t)g.nt[2] * g.nt[1]);
+    const int64_t bi = ti * kTile + w / (kTile * kTile), bj = tj * kTile + (w / kTile) % kTile,
+                  bk = tk * kTile + w % kTile;
+    const int t = threadIdx.x;
+    const int64_t ci = bi * kBrick[0] + t / (kBrick[1] * kBrick[2]);
+    const int64_t cj = bj * kBrick[1] + (t / kBrick[2]) % kBrick[1];
+    const int64_t ck = bk * kBrick[2] + t % kBrick[2];
+    if (ci >= a.n[0] || cj >= a.n[1] || ck >= a.n[2]) return -1;
+    return (ci * a.n[1] + cj) * a.n[2] + ck;
+}
+
 template <int KIND>
-__global__ void __launch_bounds__(kBlock) k_locate_centres(SampleArgs a, int32_t *__restrict__ centre) {
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride) {
+__global__ void __launch_bounds__(kBlock) k_locate_centres(SampleArgs a, int32_t *__restrict__ centre, BrickGrid bg) {
+    const bool bricks = DTFE_CENTRE_BRICKS && KIND != DTFE_SAMPLE_POINTS;
+    int64_t stride = bricks ? a.n_rows : (int64_t)gridDim.x * blockDim.x;
+    int64_t row0 = bricks ? brick_row(a, bg) : (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row0 < 0) return;
+    for (int64_t row = row0; row < a.n_rows; row += stride) {
         d3 c = row_centre<KIND>(a, row);
         Located L = locate<false>(a.rec, c, seed_cell(a.sg, c));
         int t = L.t;
@@ -1897,11 +1943,15 @@ void launch_lanes(const SampleArgs &a, const int32_t *centre, int ns, int lpr, i
 }
 
 void launch_centres(const SampleArgs &a, int32_t *centre, hipStream_t s) {
-    const dim3 g(grid_for(a.n_rows)), b(kBlock);
+    const BrickGrid bg = brick_grid(a.n);
+    const dim3 b(kBlock);
+    const dim3 gb(DTFE_CENTRE_BRICKS ? (unsigned)bg.blocks() : (unsigned)grid_for(a.n_rows));
     switch (a.kind) {
-        case DTFE_SAMPLE_GRID: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_GRID>, g, b, 0, s, a, centre); break;
-        case DTFE_SAMPLE_CONE: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_CONE>, g, b, 0, s, a, centre); break;
-        default: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_POINTS>, g, b, 0, s, a, centre); break;
+        case DTFE_SAMPLE_GRID: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_GRID>, gb, b, 0, s, a, centre, bg); break;
+        case DTFE_SAMPLE_CONE: hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_CONE>, gb, b, 0, s, a, centre, bg); break;
+        default:
+            hipLaunchKernelGGL(k_locate_centres<DTFE_SAMPLE_POINTS>, dim3(grid_for(a.n_rows)), b, 0, s, a, centre, bg);
+            break;
     }
 }
 
