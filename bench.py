@@ -47,6 +47,8 @@ def parse():
                     help="threads of the multi-threaded CPU oracle run over the whole grid")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint / forward ratio")
+    ap.add_argument("--adjoint-grid", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (calibrates the HBM byte counters)")
@@ -210,6 +212,10 @@ def main():
                 "mixed_vs_fp64_max_rel_density_err": rel}
         del o64, w64, v64
 
+    fa = None
+    if not args.no_adjoint:
+        fa = fwd_adjoint(args, dm, w, vel, prep, box)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, prep, m, box, n, mc_seed)
@@ -246,10 +252,57 @@ def main():
                      "kernel": kernel, "algorithmic_bytes": B_alg,
                      "kernel_ms": k_sample_ms},
         "fp64_path": fp64,
-        "fwd_adjoint_ratio": None,
+        "fwd_adjoint_ratio": None if fa is None else fa["ratio"],
+        "fwd_adjoint": fa,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def fwd_adjoint(args, dm, w, vel, prep, box):
+    """Forward (vertex densities, then cell-centre density and velocity
+    divergence on a 128^3 grid) against forward + adjoint (dL/d positions,
+    weights, velocities of L = sum_c w_c (rho_c - rho*_c)^2 + sum_c
+    (theta_c - theta*_c)^2 with fixed synthetic targets), same mesh."""
+    import torch
+    from dtfe_amd import api
+    dev = w.device
+    g = args.adjoint_grid
+    spec = api.GridSpec(box, (g, g, g))
+    fields = ["density", "velocity_divergence"]
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    wc = (0.5 + torch.rand(g ** 3, generator=gen, dtype=torch.float64)).to(dev, w.dtype)
+
+    def forward():
+        rho = api.vertex_density(dm, w, prep.rho_bar)
+        return rho, api.sample(dm, rho, vel, None, fields, spec, method=0)
+
+    rho, out = forward()
+    rho_t = (out["density"] * 1.1).detach()
+    th_t = (out["velocity_divergence"] * 0.9).detach()
+
+    def step():
+        rho, out = forward()
+        up = {"density": 2 * wc * (out["density"] - rho_t),
+              "velocity_divergence": 2 * (out["velocity_divergence"] - th_t)}
+        return api.adjoint(dm, rho, w, vel, prep.rho_bar, spec, up, method=0)
+
+    def timed(fn, k):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    k = max(3, args.steps)
+    t_f = timed(forward, k)
+    t_fa = timed(step, k)
+    return {"ratio": t_fa / t_f, "forward_ms": t_f, "forward_adjoint_ms": t_fa,
+            "workload": f"same mesh; {g}^3 cell-centre density + velocity divergence, FP32 fields; adjoint "
+                        f"to positions, weights and velocities (loss upstream terms computed in torch)"}
 
 
 def cpu_baseline(args, prep, m, box, n, mc_seed):

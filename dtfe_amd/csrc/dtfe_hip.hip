@@ -553,6 +553,7 @@ struct SampleArgs {
     const int4 *cells;
     const int4 *nbrs;
     int64_t nc;
+    int64_t nv;
     SeedGrid sg;
     VertexFieldsDev vf;
     unsigned fields;
@@ -825,20 +826,91 @@ __device__ __forceinline__ RowGrad<F> row_upstream(const SampleArgs &a, int64_t 
     return g;
 }
 
+// Where the adjoint's per-vertex contributions go: dL/d rho_v (1), dL/d v_v
+// (3), dL/d x_v (3).
+struct VertexGrad {
+    double r, v[3], x[3];
+};
+
+// straight to the accumulators in HBM
+struct GlobalSink {
+    template <typename F>
+    __device__ __forceinline__ void add(const SampleArgs &a, int v, const VertexGrad &d) const {
+        if (d.r != 0.0) atomic_add_f<F>(a.d_rho, v, (F)d.r);
+        if (a.d_vel)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (d.v[c] != 0.0) atomic_add_f<F>(a.d_vel, 3 * (int64_t)v + c, (F)d.v[c]);
+        if (a.d_pos)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (d.x[c] != 0.0) atomic_add_f<F>(a.d_pos, 3 * (int64_t)v + c, (F)d.x[c]);
+    }
+};
+
+// merged per vertex in an LDS hash table of the workgroup (FP64), flushed to
+// HBM once per workgroup pass; a vertex that finds no slot within a few
+// probes goes straight to HBM
+constexpr int kSinkSlots = 512;
+constexpr int kSinkVals = 7;
+
+struct LdsSink {
+    int *keys;      // [kSinkSlots], -1 = free
+    double *vals;   // [kSinkSlots][kSinkVals]
+    __device__ void clear() const {
+        for (int i = threadIdx.x; i < kSinkSlots; i += blockDim.x) {
+            keys[i] = -1;
+#pragma unroll
+            for (int q = 0; q < kSinkVals; ++q) vals[i * kSinkVals + q] = 0.0;
+        }
+    }
+    template <typename F>
+    __device__ __forceinline__ void add(const SampleArgs &a, int v, const VertexGrad &d) const {
+        unsigned slot = ((unsigned)v * 2654435761u) >> 23;  // 9 bits
+        for (int probe = 0; probe < 16; ++probe) {
+            const int k = atomicCAS(&keys[slot], -1, v);
+            if (k == -1 || k == v) {
+                double *o = vals + slot * kSinkVals;
+                if (d.r != 0.0) atomicAdd(o, d.r);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    if (d.v[c] != 0.0) atomicAdd(o + 1 + c, d.v[c]);
+                    if (d.x[c] != 0.0) atomicAdd(o + 4 + c, d.x[c]);
+                }
+                return;
+            }
+            slot = (slot + 1) & (kSinkSlots - 1);
+        }
+        GlobalSink().add<F>(a, v, d);
+    }
+    template <typename F>
+    __device__ void flush(const SampleArgs &a) const {
+        for (int i = threadIdx.x; i < kSinkSlots; i += blockDim.x) {
+            const int v = keys[i];
+            if (v < 0) continue;
+            const double *o = vals + i * kSinkVals;
+            GlobalSink().add<F>(a, v, VertexGrad{o[0], {o[1], o[2], o[3]}, {o[4], o[5], o[6]}});
+        }
+    }
+};
+
 // Scatters one sample's contribution (weight s) to the vertex accumulators.
-template <typename F, bool VEL>
+template <typename F, bool VEL, typename Sink>
 __device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Located &L, d3 p, const RowGrad<F> &g,
-                                               F s, bool value_bug) {
+                                               F s, bool value_bug, const Sink &sink) {
     const int id[4] = {L.v.x, L.v.y, L.v.z, L.v.w};
+    VertexGrad out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = VertexGrad{0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double Ai[3][3];
     bool ok = ref_inverse(L.X, Ai);
     if (!ok) {
         // the reference's degenerate rule: every field is vertex 0's value
-        if (g.gr != (F)0) atomic_add_f<F>(a.d_rho, id[0], s * g.gr);
-        if (VEL && a.d_vel)
+        out[0].r = (double)(s * g.gr);
+        if (VEL)
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (g.gv[c] != (F)0) atomic_add_f<F>(a.d_vel, 3 * (int64_t)id[0] + c, s * g.gv[c]);
+            for (int c = 0; c < 3; ++c) out[0].v[c] = (double)(s * g.gv[c]);
+        sink.template add<F>(a, id[0], out[0]);
         return;
     }
     // grad(lambda_i): column i-1 of A^-1 for i >= 1, minus their sum for i = 0
@@ -866,7 +938,7 @@ __device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Locate
         for (int b = 0; b < 3; ++b) grho[b] = r[0] * gl[0][b] + r[1] * gl[1][b] + r[2] * gl[2][b] + r[3] * gl[3][b];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            atomic_add_f<F>(a.d_rho, id[i], s * g.gr * lam[i]);
+            out[i].r = (double)(s * g.gr * lam[i]);
 #pragma unroll
             for (int b = 0; b < 3; ++b) dX[i][b] -= s * g.gr * lam[i] * grho[b];
         }
@@ -900,7 +972,7 @@ __device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Locate
                         else acc += g.gv[ca] * lam[i];
                     }
                     if (gG_any) acc += g.gG[3 * ca] * gl[i][0] + g.gG[3 * ca + 1] * gl[i][1] + g.gG[3 * ca + 2] * gl[i][2];
-                    if (acc != (F)0 && a.d_vel) atomic_add_f<F>(a.d_vel, 3 * (int64_t)id[i] + ca, s * acc);
+                    out[i].v[ca] = (double)(s * acc);
                 }
             // dL/dX_k
             F Gtgv[3];  // G^T gv
@@ -933,22 +1005,23 @@ __device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Locate
             }
         }
     }
-    if (a.d_pos)
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 4; ++k) {
 #pragma unroll
-            for (int b = 0; b < 3; ++b)
-                if (dX[k][b] != (F)0) atomic_add_f<F>(a.d_pos, 3 * (int64_t)id[k] + b, dX[k][b]);
+        for (int b = 0; b < 3; ++b) out[k].x[b] = (double)dX[k][b];
+        sink.template add<F>(a, id[k], out[k]);
+    }
 }
 
 // Backward of one row. Returns false (nothing scattered) when a sign could
 // not be certified; the exact kernel then redoes the row.
-template <typename F, bool VEL, bool EXACT, typename Words>
-__device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, const Words &words) {
+template <typename F, bool VEL, bool EXACT, typename Words, typename Sink = GlobalSink, int KIND = -1>
+__device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, const Words &words,
+                                            const Sink &sink = Sink()) {
     const bool value_bug = (a.flags & DTFE_COMPAT_VALUE_BUG) != 0;
     RowGrad<F> g = row_upstream<F, VEL>(a, row);
     if (!g.any_g) return true;
-    d3 centre = row_centre(a, row);
+    d3 centre = row_centre<KIND>(a, row);
     int start = seed_cell(a.sg, centre);
     Located Lc = locate<EXACT>(a.rec, centre, start);
     if (!EXACT && Lc.t == -3) return false;
@@ -957,7 +1030,7 @@ __device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, co
         Lc.t = -1;
     }
     if (a.method == DTFE_METHOD_CENTRE) {
-        if (Lc.t >= 0) scatter_sample<F, VEL>(a, Lc, centre, g, (F)1, value_bug);
+        if (Lc.t >= 0) scatter_sample<F, VEL>(a, Lc, centre, g, (F)1, value_bug, sink);
         return true;
     }
     int hint = Lc.t >= 0 ? Lc.t : start;
@@ -987,7 +1060,7 @@ __device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, co
             atomicAdd(a.status, 1);
             continue;
         }
-        if (L.t >= 0) scatter_sample<F, VEL>(a, L, p, g, s, value_bug);
+        if (L.t >= 0) scatter_sample<F, VEL>(a, L, p, g, s, value_bug, sink);
     }
     return true;
 }
@@ -1267,6 +1340,39 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_plain(SampleArgs a) {
     NoWords nw;
     for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride)
         if (!run_row<F, VEL, NS, false, ADJ>(a, row, nw)) defer_row(a, row, nw);
+}
+
+// Backward of rows without random draws (cell centres, equidistant
+// averages): grids and cones brick by brick, so that a workgroup's rows share
+// vertices, whose contributions are merged in LDS before the HBM atomics.
+#ifndef DTFE_ADJ_MIN_WAVES
+#define DTFE_ADJ_MIN_WAVES 2
+#endif
+template <typename F, bool VEL, int KIND>
+__global__ void __launch_bounds__(kBlock, DTFE_ADJ_MIN_WAVES) k_adjoint_rows(SampleArgs a, BrickGrid bg) {
+    __shared__ int keys[kSinkSlots];
+    __shared__ double vals[kSinkSlots * kSinkVals];
+    const LdsSink sink{keys, vals};
+    NoWords nw;
+    if (KIND != DTFE_SAMPLE_POINTS) {
+        sink.clear();
+        __syncthreads();
+        const int64_t row = brick_row(a, bg);
+        if (row >= 0 && !adjoint_row<F, VEL, false, NoWords, LdsSink, KIND>(a, row, nw, sink)) defer_row(a, row, nw);
+        __syncthreads();
+        sink.template flush<F>(a);
+        return;
+    }
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.n_rows; base += (int64_t)gridDim.x * blockDim.x) {
+        sink.clear();
+        __syncthreads();
+        const int64_t row = base + threadIdx.x;
+        if (row < a.n_rows && !adjoint_row<F, VEL, false, NoWords, LdsSink, KIND>(a, row, nw, sink))
+            defer_row(a, row, nw);
+        __syncthreads();
+        sink.template flush<F>(a);
+        __syncthreads();
+    }
 }
 
 // Monte-Carlo rows: workgroup b owns rows [b*rows_per_block, ...) and
@@ -1551,29 +1657,101 @@ __global__ void k_adj_vertex(const F *__restrict__ w, const F *__restrict__ rho,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// The same from rho alone: rho_v = 4 w_v / (rho_bar W_v) > 0 gives
+// W_v = 4 w_v / (rho_bar rho_v), so d rho_v / d w_v = rho_v / w_v and
+// d rho_v / d W_v = -rho_v / W_v. A vertex with rho_v = 0 is a hull vertex
+// (rho fixed at 0: no gradient) or has w_v = 0, whose d rho_v / d w_v needs
+// W_v: such a vertex raises *need_W and the caller takes the volume path.
+template <typename F>
+__global__ void k_adj_vertex_rho(const F *__restrict__ w, const F *__restrict__ rho, const F *__restrict__ d_rho,
+                                 int64_t nv, double rho_bar, F *__restrict__ d_w, double *__restrict__ dW,
+                                 double *__restrict__ partial, int *__restrict__ need_W) {
+    __shared__ double red[kBlock];
+    double acc = 0.0;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+        const double g = (double)d_rho[v];
+        const double r = (double)rho[v];
+        dW[v] = 0.0;
+        if (g == 0.0) continue;
+        const double wv = (double)w[v];
+        if (!(r > 0.0)) {
+            if (wv == 0.0) atomicOr(need_W, 1);
+            continue;
+        }
+        const double Wv = 4.0 * wv / (rho_bar * r);
+        if (d_w) d_w[v] += (F)(g * r / wv);
+        dW[v] = -g * r / Wv;
+        acc += -g * r / rho_bar;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // dV_t/dX_1 = (e2 x e3)/6, dV_t/dX_2 = (e3 x e1)/6, dV_t/dX_3 = (e1 x e2)/6,
 // dV_t/dX_0 = -(their sum), with e_k = X_k - X_0.
+// dL/dX from dL/dW: each tetrahedron adds dV * dV/dX_k to its four vertices.
+// As in k_cell_volumes, a workgroup's 4 x 256 vertex contributions are first
+// merged in an LDS hash table (FP64), then each distinct vertex gets three
+// atomics in HBM.
+constexpr int kAdjSlots = 1024;  // >= 4 * kBlock keys fit
+
 template <typename F>
-__global__ void k_adj_tets(const double *__restrict__ pos, const int4 *__restrict__ cells, int64_t nc,
-                           const double *__restrict__ dW, F *__restrict__ d_pos) {
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += stride) {
-        int4 c = cells[t];
-        double dV = (dW[c.x] + dW[c.y]) + (dW[c.z] + dW[c.w]);
-        if (dV == 0.0) continue;
-        d3 x0 = load_pos(pos, c.x), x1 = load_pos(pos, c.y), x2 = load_pos(pos, c.z), x3 = load_pos(pos, c.w);
-        d3 e1 = sub(x1, x0), e2 = sub(x2, x0), e3 = sub(x3, x0);
-        d3 g1 = cross(e2, e3), g2 = cross(e3, e1), g3 = cross(e1, e2);
-        const double k = dV / 6.0;
-        d3 g0{-(g1.x + g2.x + g3.x), -(g1.y + g2.y + g3.y), -(g1.z + g2.z + g3.z)};
-        const int id[4] = {c.x, c.y, c.z, c.w};
-        const d3 gg[4] = {g0, g1, g2, g3};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            atomicAdd(d_pos + 3 * (int64_t)id[q] + 0, (F)(k * gg[q].x));
-            atomicAdd(d_pos + 3 * (int64_t)id[q] + 1, (F)(k * gg[q].y));
-            atomicAdd(d_pos + 3 * (int64_t)id[q] + 2, (F)(k * gg[q].z));
+__global__ void __launch_bounds__(kBlock) k_adj_tets(const double *__restrict__ pos, const int4 *__restrict__ cells,
+                                                     int64_t nc, const double *__restrict__ dW,
+                                                     F *__restrict__ d_pos) {
+    __shared__ int keys[kAdjSlots];
+    __shared__ double vals[kAdjSlots][3];
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
+        for (int i = threadIdx.x; i < kAdjSlots; i += blockDim.x) {
+            keys[i] = -1;
+            vals[i][0] = vals[i][1] = vals[i][2] = 0.0;
         }
+        __syncthreads();
+        const int64_t t = base + threadIdx.x;
+        if (t < nc) {
+            const int4 c = cells[t];
+            const double dV = (dW[c.x] + dW[c.y]) + (dW[c.z] + dW[c.w]);
+            if (dV != 0.0) {
+                d3 x0 = load_pos(pos, c.x), x1 = load_pos(pos, c.y), x2 = load_pos(pos, c.z), x3 = load_pos(pos, c.w);
+                d3 e1 = sub(x1, x0), e2 = sub(x2, x0), e3 = sub(x3, x0);
+                d3 g1 = cross(e2, e3), g2 = cross(e3, e1), g3 = cross(e1, e2);
+                const double k = dV / 6.0;
+                d3 g0{-(g1.x + g2.x + g3.x), -(g1.y + g2.y + g3.y), -(g1.z + g2.z + g3.z)};
+                const int id[4] = {c.x, c.y, c.z, c.w};
+                const d3 gg[4] = {g0, g1, g2, g3};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int v = id[q];
+                    unsigned slot = ((unsigned)v * 2654435761u) >> 22;  // 10 bits
+                    for (int probe = 0; probe < kAdjSlots; ++probe) {
+                        const int kk = atomicCAS(&keys[slot], -1, v);
+                        if (kk == -1 || kk == v) {
+                            atomicAdd(&vals[slot][0], k * gg[q].x);
+                            atomicAdd(&vals[slot][1], k * gg[q].y);
+                            atomicAdd(&vals[slot][2], k * gg[q].z);
+                            break;
+                        }
+                        slot = (slot + 1) & (kAdjSlots - 1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kAdjSlots; i += blockDim.x) {
+            const int v = keys[i];
+            if (v >= 0) {
+                atomicAdd(d_pos + 3 * (int64_t)v + 0, (F)vals[i][0]);
+                atomicAdd(d_pos + 3 * (int64_t)v + 1, (F)vals[i][1]);
+                atomicAdd(d_pos + 3 * (int64_t)v + 2, (F)vals[i][2]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1896,7 +2074,17 @@ void launch_sample(const SampleArgs &a, int which, int mc, int grid, size_t lds,
         hipLaunchKernelGGL((k_sample_exact_all<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), mc ? lds : 0, s, a);
     else if (mc)
         hipLaunchKernelGGL((k_sample_mc<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), lds, s, a);
-    else
+    else if (ADJ) {
+        const BrickGrid bg = brick_grid(a.n);
+        const dim3 gb((unsigned)bg.blocks());
+        switch (a.kind) {
+            case DTFE_SAMPLE_GRID: hipLaunchKernelGGL((k_adjoint_rows<F, VEL, DTFE_SAMPLE_GRID>), gb, dim3(kBlock), 0, s, a, bg); break;
+            case DTFE_SAMPLE_CONE: hipLaunchKernelGGL((k_adjoint_rows<F, VEL, DTFE_SAMPLE_CONE>), gb, dim3(kBlock), 0, s, a, bg); break;
+            default:
+                hipLaunchKernelGGL((k_adjoint_rows<F, VEL, DTFE_SAMPLE_POINTS>), dim3(grid), dim3(kBlock), 0, s, a, bg);
+                break;
+        }
+    } else
         hipLaunchKernelGGL((k_sample_plain<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
@@ -2204,8 +2392,19 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     ctx->last_ms[1] = 0.0;
     ctx->last_deferred = st[1];
     if (st[2]) {
-        // queue overflow: redo everything with exact signs (the adjoint never
-        // scatters a row before all of its walks are certified)
+        // queue overflow: redo everything with exact signs; the adjoint first
+        // drops what the certified rows already scattered
+        if (adj) {
+            const int64_t nv = a.nv;
+            auto clear = [&](void *p, int64_t n) {
+                if (!p) return;
+                if (f64) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (double *)p, n, 0.0);
+                else hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (float *)p, n, 0.0f);
+            };
+            clear(a.d_rho, nv);
+            clear(a.d_vel, 3 * nv);
+            clear(a.d_pos, 3 * nv);
+        }
         launch_any(a, f64, vel, ns, adj, 2, is_mc, grid, lds, 0, ctx->stream);
     } else if (st[1] > 0) {
         HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -2392,6 +2591,7 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
     a.cells = (const int4 *)m->cells;
     a.nbrs = (const int4 *)m->neighbors;
     a.nc = m->n_cells;
+    a.nv = m->n_vertices;
     a.vf = VertexFieldsDev{vf->density, vf->velocity, vf->scalar, vf->n_scalar};
     a.fields = fields;
     a.flags = flags;
@@ -2557,6 +2757,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     a.cells = (const int4 *)m->cells;
     a.nbrs = (const int4 *)m->neighbors;
     a.nc = nc;
+    a.nv = nv;
     a.vf = VertexFieldsDev{vf->density, vf->velocity, nullptr, 0};
     a.flags = flags;
     a.kind = smp->kind;
@@ -2581,28 +2782,47 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     }
     // vertex density chain
     if (up->density && (gr->weight || gr->position)) {
-        if ((rc = ensure(ctx, ctx->W, nv * sizeof(double)))) return rc;
-        if ((rc = ensure(ctx, ctx->hull, nv))) return rc;
         if ((rc = ensure(ctx, ctx->adj_dW, nv * sizeof(double)))) return rc;
         const int parts = std::min(1024, grid_for(nv));
-        if ((rc = ensure(ctx, ctx->adj_partial, parts * sizeof(double)))) return rc;
-        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->W.p, nv,
-                           0.0);
-        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (uint8_t *)ctx->hull.p,
-                           nv, (uint8_t)0);
-        hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
-                           (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
-                           (uint8_t *)ctx->hull.p);
+        if ((rc = ensure(ctx, ctx->adj_partial, parts * sizeof(double) + 64))) return rc;
+        int *need_W = (int *)((char *)ctx->adj_partial.p + parts * sizeof(double));
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, need_W, (int64_t)1, 0);
         if (f64)
-            hipLaunchKernelGGL(k_adj_vertex<double>, dim3(parts), dim3(kBlock), 0, ctx->stream,
-                               (const double *)weight, (const double *)vf->density, (const double *)ctx->W.p,
-                               (const uint8_t *)ctx->hull.p, (const double *)ctx->adj_rho.p, nv, rho_bar,
-                               (double *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+            hipLaunchKernelGGL(k_adj_vertex_rho<double>, dim3(parts), dim3(kBlock), 0, ctx->stream,
+                               (const double *)weight, (const double *)vf->density, (const double *)ctx->adj_rho.p,
+                               nv, rho_bar, (double *)gr->weight, (double *)ctx->adj_dW.p,
+                               (double *)ctx->adj_partial.p, need_W);
         else
-            hipLaunchKernelGGL(k_adj_vertex<float>, dim3(parts), dim3(kBlock), 0, ctx->stream,
-                               (const float *)weight, (const float *)vf->density, (const double *)ctx->W.p,
-                               (const uint8_t *)ctx->hull.p, (const float *)ctx->adj_rho.p, nv, rho_bar,
-                               (float *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+            hipLaunchKernelGGL(k_adj_vertex_rho<float>, dim3(parts), dim3(kBlock), 0, ctx->stream,
+                               (const float *)weight, (const float *)vf->density, (const float *)ctx->adj_rho.p,
+                               nv, rho_bar, (float *)gr->weight, (double *)ctx->adj_dW.p,
+                               (double *)ctx->adj_partial.p, need_W);
+        int need = 0;
+        HIP_TRY(hipMemcpyAsync(&need, need_W, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (need) {
+            // a zero-weight interior vertex: the incident volumes from the mesh
+            if ((rc = ensure(ctx, ctx->W, nv * sizeof(double)))) return rc;
+            if ((rc = ensure(ctx, ctx->hull, nv))) return rc;
+            if (gr->weight) zero(gr->weight, nv);
+            hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->W.p,
+                               nv, 0.0);
+            hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                               (uint8_t *)ctx->hull.p, nv, (uint8_t)0);
+            hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+                               (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
+                               (uint8_t *)ctx->hull.p);
+            if (f64)
+                hipLaunchKernelGGL(k_adj_vertex<double>, dim3(parts), dim3(kBlock), 0, ctx->stream,
+                                   (const double *)weight, (const double *)vf->density, (const double *)ctx->W.p,
+                                   (const uint8_t *)ctx->hull.p, (const double *)ctx->adj_rho.p, nv, rho_bar,
+                                   (double *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+            else
+                hipLaunchKernelGGL(k_adj_vertex<float>, dim3(parts), dim3(kBlock), 0, ctx->stream,
+                                   (const float *)weight, (const float *)vf->density, (const double *)ctx->W.p,
+                                   (const uint8_t *)ctx->hull.p, (const float *)ctx->adj_rho.p, nv, rho_bar,
+                                   (float *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+        }
         if (gr->position) {
             if (f64)
                 hipLaunchKernelGGL(k_adj_tets<double>, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,

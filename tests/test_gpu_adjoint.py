@@ -31,14 +31,23 @@ def _grid(prob, n):
     return np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
 
 
-@pytest.mark.parametrize("value_bug", [True, False])
-def test_adjoint_matches_fp64_autograd(gpu, value_bug):
+@pytest.mark.parametrize("value_bug,zero_weights", [(True, False), (False, False), (True, True)])
+def test_adjoint_matches_fp64_autograd(gpu, value_bug, zero_weights):
+    """zero_weights: every 7th vertex weighs nothing, so its density is 0
+    although it is not on the hull (the adjoint then needs the incident
+    volumes, not just the densities)."""
     from dtfe_amd import api
     prob = P.clustered_problem()
     m = prob.mesh
     n = (12, 12, 12)
     spec = api.GridSpec(prob.region, n)
-    dm, w, v, rho = _setup(prob, torch.float64)
+    wts = prob.w.copy()
+    if zero_weights:
+        wts[::7] = 0.0
+    dm = api.DeviceMesh.from_mesh(m, "cuda:0")
+    w = torch.from_numpy(wts).to("cuda:0", torch.float64)
+    v = torch.from_numpy(prob.vel).to("cuda:0", torch.float64)
+    rho = api.vertex_density(dm, w, prob.rho_bar)
     fields = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity"]
     out = api.sample(dm, rho, v, None, fields, spec, method=0, value_bug=value_bug, want_cells=True)
     located = out["located_cell"].cpu().numpy().astype(np.int64)
@@ -48,9 +57,9 @@ def test_adjoint_matches_fp64_autograd(gpu, value_bug):
     g, g_rb = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=0, value_bug=value_bug)
 
     pos = torch.from_numpy(m.pos).clone().requires_grad_(True)
-    wt = torch.from_numpy(prob.w).clone().requires_grad_(True)
+    wt = torch.from_numpy(wts).clone().requires_grad_(True)
     vt = torch.from_numpy(prob.vel).clone().requires_grad_(True)
-    _, _, hull = O.vertex_density(m, prob.w, prob.rho_bar)
+    _, _, hull = O.vertex_density(m, wts, prob.rho_bar)
     rb = torch.tensor(prob.rho_bar, dtype=torch.float64, requires_grad=True)
     d, vel, G = R.forward(pos, wt, vt, torch.from_numpy(m.cells).long(), torch.from_numpy(hull),
                           torch.from_numpy(located), torch.from_numpy(_grid(prob, n)), rb, value_bug)
@@ -65,7 +74,7 @@ def test_adjoint_matches_fp64_autograd(gpu, value_bug):
     for name, ref in (("position", pos.grad), ("weight", wt.grad), ("velocity", vt.grad)):
         got = g[name].cpu()
         err = (got - ref).abs().max().item() / ref.abs().max().item()
-        print(f"adjoint bug={value_bug} {name}: max rel err {err:.3e}")
+        print(f"adjoint bug={value_bug} zero_w={zero_weights} {name}: max rel err {err:.3e}")
         assert err < 1e-9, name
     err_rb = abs(g_rb - rb.grad.item()) / abs(rb.grad.item())
     print(f"adjoint d rho_bar rel err {err_rb:.3e}")
