@@ -87,7 +87,7 @@ class Outputs(C.Structure):
 class Upstream(C.Structure):
     _fields_ = [("density", C.c_void_p), ("velocity", C.c_void_p), ("velocity_gradient", C.c_void_p),
                 ("velocity_divergence", C.c_void_p), ("velocity_shear", C.c_void_p),
-                ("velocity_vorticity", C.c_void_p)]
+                ("velocity_vorticity", C.c_void_p), ("start_cell", C.c_void_p)]
 
 
 class VertexGrads(C.Structure):

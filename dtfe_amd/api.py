@@ -275,11 +275,12 @@ def _spec_struct(spec, keep):
 def adjoint(mesh: DeviceMesh, rho: torch.Tensor, weight: torch.Tensor, vel: Optional[torch.Tensor],
             rho_bar: float, spec, upstream: Dict[str, torch.Tensor], method: int = 0, samples: int = 1,
             seed: int = 0, part_no: int = -1, arg_order: int = 0, value_bug: bool = True,
-            want=("position", "weight", "velocity")):
+            want=("position", "weight", "velocity"), start_cells: Optional[torch.Tensor] = None):
     """Reverse mode of vertex_density + sample at fixed tessellation.
 
     upstream maps output field names to dL/d(field) (same shapes as the
-    outputs of `sample`). Returns (grads dict with 'position' [nv, 3],
+    outputs of `sample`). start_cells: the forward's `located_cell` for the
+    same rows (optional), where the walks then start. Returns (grads dict with 'position' [nv, 3],
     'weight' [nv], 'velocity' [nv, 3], and dL/d rho_bar as a float)."""
     ctx = context(mesh.device)
     dtype = rho.dtype
@@ -294,6 +295,10 @@ def adjoint(mesh: DeviceMesh, rho: torch.Tensor, weight: torch.Tensor, vel: Opti
             g = upstream[f].to(dtype).contiguous()
             keep.append(g)
             setattr(up, f, g.data_ptr())
+    if start_cells is not None:
+        sc = start_cells.to(torch.int32).contiguous()
+        keep.append(sc)
+        up.start_cell = sc.data_ptr()
     out = {}
     gr = N.VertexGrads()
     if "position" in want:

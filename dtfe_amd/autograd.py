@@ -2,9 +2,10 @@
 
 Forward: vertex densities (rho_v = 4 w_v / (rho_bar W_v)) and the sampled
 fields of `api.sample`. Backward: the hand-written adjoint kernels
-(`api.adjoint`), which re-locate the sample points and recompute their
-barycentric coordinates. The tessellation is held fixed (gradients at fixed
-topology), as are the Monte-Carlo sample points.
+(`api.adjoint`), which re-locate the sample points (starting from the cells
+the forward located) and recompute their barycentric coordinates. The
+tessellation is held fixed (gradients at fixed topology), as are the
+Monte-Carlo sample points.
 
 `dtfe_fields` is the particle-level entry: periodic images are gathered from
 the particles with an index map, so torch folds the image gradients back to
@@ -26,11 +27,15 @@ class _DTFESample(torch.autograd.Function):
     def forward(ctx, pos, weight, vel, rho_bar_t, mesh_cells, mesh_nbrs, spec, fields, method, samples, seed,
                 value_bug):
         mesh = api.DeviceMesh(pos=pos.detach().contiguous(), cells=mesh_cells, nbrs=mesh_nbrs)
+        if mesh.pos.shape[0] >= 4 and mesh.cells.shape[0] > 0:
+            mesh.build_locator()  # these positions, for the forward and its backward
         rho_bar = float(rho_bar_t.item())
         rho = api.vertex_density(mesh, weight.detach(), rho_bar)
         out = api.sample(mesh, rho, None if vel is None else vel.detach(), None, fields, spec, method=method,
-                         samples=samples, seed=seed, value_bug=value_bug)
+                         samples=samples, seed=seed, value_bug=value_bug, want_cells=True)
         ctx.save_for_backward(pos, weight, rho, *(() if vel is None else (vel,)))
+        ctx.locator = mesh.locator
+        ctx.located = out["located_cell"]
         ctx.has_vel = vel is not None
         ctx.meta = (mesh_cells, mesh_nbrs, spec, list(fields), method, samples, seed, value_bug, rho_bar)
         return tuple(out[f] for f in fields)
@@ -41,10 +46,11 @@ class _DTFESample(torch.autograd.Function):
         pos, weight, rho = saved[:3]
         vel = saved[3] if ctx.has_vel else None
         cells, nbrs, spec, fields, method, samples, seed, value_bug, rho_bar = ctx.meta
-        mesh = api.DeviceMesh(pos=pos.detach().contiguous(), cells=cells, nbrs=nbrs)
+        mesh = api.DeviceMesh(pos=pos.detach().contiguous(), cells=cells, nbrs=nbrs, locator=ctx.locator)
         upstream = {f: g for f, g in zip(fields, grads) if g is not None}
         g, g_rho_bar = api.adjoint(mesh, rho, weight.detach(), None if vel is None else vel.detach(), rho_bar, spec,
-                                   upstream, method=method, samples=samples, seed=seed, value_bug=value_bug)
+                                   upstream, method=method, samples=samples, seed=seed, value_bug=value_bug,
+                                   start_cells=ctx.located)
         g_pos = g.get("position")
         g_w = g.get("weight")
         g_v = g.get("velocity") if ctx.has_vel else None

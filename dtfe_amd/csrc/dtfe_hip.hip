@@ -581,6 +581,7 @@ struct SampleArgs {
     int defer_cap;
     // adjoint: upstream gradients per row and per-vertex accumulators
     const void *g_density, *g_velocity, *g_grad, *g_div, *g_shear, *g_vort;
+    const int32_t *start_cell;  // optional walk start per row (the forward's located cells)
     void *d_rho;   // [nv] dL/d rho_v
     void *d_vel;   // [nv][3]
     void *d_pos;   // [nv][3]
@@ -894,23 +895,22 @@ struct LdsSink {
     }
 };
 
-// Scatters one sample's contribution (weight s) to the vertex accumulators.
+// Scatters one sample's contribution (weight s) to the vertex accumulators,
+// one vertex at a time (the per-vertex terms share grad(lambda), grad(rho),
+// G and the upstream gradients).
 template <typename F, bool VEL, typename Sink>
 __device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Located &L, d3 p, const RowGrad<F> &g,
                                                F s, bool value_bug, const Sink &sink) {
     const int id[4] = {L.v.x, L.v.y, L.v.z, L.v.w};
-    VertexGrad out[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) out[k] = VertexGrad{0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double Ai[3][3];
     bool ok = ref_inverse(L.X, Ai);
     if (!ok) {
         // the reference's degenerate rule: every field is vertex 0's value
-        out[0].r = (double)(s * g.gr);
+        VertexGrad d{(double)(s * g.gr), {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
         if (VEL)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) out[0].v[c] = (double)(s * g.gv[c]);
-        sink.template add<F>(a, id[0], out[0]);
+            for (int c = 0; c < 3; ++c) d.v[c] = (double)(s * g.gv[c]);
+        sink.template add<F>(a, id[0], d);
         return;
     }
     // grad(lambda_i): column i-1 of A^-1 for i >= 1, minus their sum for i = 0
@@ -926,90 +926,85 @@ __device__ __forceinline__ void scatter_sample(const SampleArgs &a, const Locate
     F lam[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) lam[i] = (F)(L.O[i] / osum);
-    F dX[4][3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) dX[k][0] = dX[k][1] = dX[k][2] = (F)0;
-    if (g.gr != (F)0) {
+    const bool g_rho = g.gr != (F)0;
+    F grho[3] = {0, 0, 0};
+    if (g_rho) {
         F r[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) r[i] = ldf<F>(a.vf.rho, id[i]);
-        F grho[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) grho[b] = r[0] * gl[0][b] + r[1] * gl[1][b] + r[2] * gl[2][b] + r[3] * gl[3][b];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            out[i].r = (double)(s * g.gr * lam[i]);
-#pragma unroll
-            for (int b = 0; b < 3; ++b) dX[i][b] -= s * g.gr * lam[i] * grho[b];
-        }
     }
+    bool gv_any = false, gG_any = false;
+    F G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // G[a][b] = d v_a / d x_b
+    F Gtgv[3] = {0, 0, 0};                          // G^T gv
+    F dz = 0;
     if (VEL) {
-        bool gv_any = g.gv[0] != (F)0 || g.gv[1] != (F)0 || g.gv[2] != (F)0;
-        bool gG_any = false;
+        gv_any = g.gv[0] != (F)0 || g.gv[1] != (F)0 || g.gv[2] != (F)0;
 #pragma unroll
         for (int c = 0; c < 9; ++c) gG_any = gG_any || g.gG[c] != (F)0;
         if (gv_any || gG_any) {
-            F v[4][3];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) v[i][c] = ldf<F>(a.vf.vel, 3 * (int64_t)id[i] + c);
-            F G[3][3];  // G[a][b] = d v_a / d x_b
-#pragma unroll
-            for (int ca = 0; ca < 3; ++ca)
-#pragma unroll
-                for (int b = 0; b < 3; ++b)
-                    G[ca][b] = v[0][ca] * gl[0][b] + v[1][ca] * gl[1][b] + v[2][ca] * gl[2][b] + v[3][ca] * gl[3][b];
-            F dz = (F)(p.z - L.X[0].z);
-            // dL/dv_i
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int ca = 0; ca < 3; ++ca) {
-                    F acc = 0;
-                    if (gv_any) {
-                        if (value_bug) acc += g.gv[ca] * ((i == 0 ? (F)1 : (F)0) + gl[i][2] * dz);
-                        else acc += g.gv[ca] * lam[i];
-                    }
-                    if (gG_any) acc += g.gG[3 * ca] * gl[i][0] + g.gG[3 * ca + 1] * gl[i][1] + g.gG[3 * ca + 2] * gl[i][2];
-                    out[i].v[ca] = (double)(s * acc);
+                    const F v = ldf<F>(a.vf.vel, 3 * (int64_t)id[i] + ca);
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) G[ca][b] += v * gl[i][b];
                 }
-            // dL/dX_k
-            F Gtgv[3];  // G^T gv
 #pragma unroll
             for (int b = 0; b < 3; ++b) Gtgv[b] = G[0][b] * g.gv[0] + G[1][b] * g.gv[1] + G[2][b] * g.gv[2];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (gv_any) {
-                    if (value_bug) {
-                        // v_a = v_0a + G_az (p_z - x0_z)
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) dX[k][b] -= s * dz * gl[k][2] * Gtgv[b];
-                    } else {
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) dX[k][b] -= s * lam[k] * Gtgv[b];
-                    }
-                }
-                if (gG_any) {
-                    F q[3];  // gG grad(lambda_k)
-#pragma unroll
-                    for (int ca = 0; ca < 3; ++ca)
-                        q[ca] = g.gG[3 * ca] * gl[k][0] + g.gG[3 * ca + 1] * gl[k][1] + g.gG[3 * ca + 2] * gl[k][2];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) dX[k][c] -= s * (G[0][c] * q[0] + G[1][c] * q[1] + G[2][c] * q[2]);
-                }
-            }
-            if (gv_any && value_bug) {
-                // the x0_z of the reference's formula moves with vertex 0
-                dX[0][2] -= s * (g.gv[0] * G[0][2] + g.gv[1] * G[1][2] + g.gv[2] * G[2][2]);
-            }
+            dz = (F)(p.z - L.X[0].z);
         }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
+        VertexGrad d{0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        F dX[3] = {0, 0, 0};
+        if (g_rho) {
+            d.r = (double)(s * g.gr * lam[k]);
 #pragma unroll
-        for (int b = 0; b < 3; ++b) out[k].x[b] = (double)dX[k][b];
-        sink.template add<F>(a, id[k], out[k]);
+            for (int b = 0; b < 3; ++b) dX[b] -= s * g.gr * lam[k] * grho[b];
+        }
+        if (VEL && (gv_any || gG_any)) {
+            // dL/dv_k
+#pragma unroll
+            for (int ca = 0; ca < 3; ++ca) {
+                F acc = 0;
+                if (gv_any) {
+                    if (value_bug) acc += g.gv[ca] * ((k == 0 ? (F)1 : (F)0) + gl[k][2] * dz);
+                    else acc += g.gv[ca] * lam[k];
+                }
+                if (gG_any) acc += g.gG[3 * ca] * gl[k][0] + g.gG[3 * ca + 1] * gl[k][1] + g.gG[3 * ca + 2] * gl[k][2];
+                d.v[ca] = (double)(s * acc);
+            }
+            // dL/dX_k
+            if (gv_any) {
+                if (value_bug) {
+                    // v_a = v_0a + G_az (p_z - x0_z)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) dX[b] -= s * dz * gl[k][2] * Gtgv[b];
+                } else {
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) dX[b] -= s * lam[k] * Gtgv[b];
+                }
+            }
+            if (gG_any) {
+                F q[3];  // gG grad(lambda_k)
+#pragma unroll
+                for (int ca = 0; ca < 3; ++ca)
+                    q[ca] = g.gG[3 * ca] * gl[k][0] + g.gG[3 * ca + 1] * gl[k][1] + g.gG[3 * ca + 2] * gl[k][2];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dX[c] -= s * (G[0][c] * q[0] + G[1][c] * q[1] + G[2][c] * q[2]);
+            }
+            if (k == 0 && gv_any && value_bug) {
+                // the x0_z of the reference's formula moves with vertex 0
+                dX[2] -= s * (g.gv[0] * G[0][2] + g.gv[1] * G[1][2] + g.gv[2] * G[2][2]);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) d.x[b] = (double)dX[b];
+        sink.template add<F>(a, id[k], d);
     }
 }
 
@@ -1022,7 +1017,9 @@ __device__ __forceinline__ bool adjoint_row(const SampleArgs &a, int64_t row, co
     RowGrad<F> g = row_upstream<F, VEL>(a, row);
     if (!g.any_g) return true;
     d3 centre = row_centre<KIND>(a, row);
-    int start = seed_cell(a.sg, centre);
+    // the forward's located cell of the row, when the caller passes it
+    const int hint0 = a.start_cell ? a.start_cell[row] : -1;
+    int start = hint0 >= 0 ? hint0 : seed_cell(a.sg, centre);
     Located Lc = locate<EXACT>(a.rec, centre, start);
     if (!EXACT && Lc.t == -3) return false;
     if (Lc.t == -2) {
@@ -2770,6 +2767,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     a.g_div = up->velocity_divergence;
     a.g_shear = up->velocity_shear;
     a.g_vort = up->velocity_vorticity;
+    a.start_cell = up->start_cell;
     a.d_rho = ctx->adj_rho.p;
     a.d_vel = gr->velocity;
     a.d_pos = gr->position;

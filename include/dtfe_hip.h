@@ -182,7 +182,8 @@ int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, i
  * Reverse mode of dtfe_hip_vertex_density + dtfe_hip_sample at fixed
  * tessellation: given dL/d(output rows), returns dL/d(vertex positions,
  * weights, velocities). Sample points are re-located and their barycentric
- * coordinates recomputed (nothing is stored by the forward). Outputs are
+ * coordinates recomputed (the forward may hand over its located cells as
+ * walk starts, nothing else). Outputs are
  * overwritten; NULL outputs are skipped. rho_bar is held fixed inside the
  * chain and dL/d rho_bar is returned, so a caller whose rho_bar is
  * sum(w)/35000^3 adds rho_bar_grad / 35000^3 to every particle weight. */
@@ -193,6 +194,10 @@ typedef struct {
     const void *velocity_divergence; /* [n]    */
     const void *velocity_shear;      /* [n][5] */
     const void *velocity_vorticity;  /* [n][3] */
+    /* Optional [n] located_cell of the forward call on the same rows: each
+     * row's walk starts there (cells, not barycentric coordinates, are
+     * kept); NULL: walks start from the locator's seed grid. */
+    const int32_t *start_cell;
 } dtfe_upstream;
 
 typedef struct {

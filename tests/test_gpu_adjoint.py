@@ -157,3 +157,24 @@ def test_adjoint_central_differences(gpu):
             checked += 1
     print(f"central differences checked: {checked}")
     assert checked >= 20
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_adjoint_start_cells(gpu, method):
+    """Starting the adjoint's walks at the forward's located cells changes
+    nothing but where the walks begin."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    spec = api.GridSpec(prob.region, (11, 10, 9))
+    dm, w, v, rho = _setup(prob, torch.float64)
+    fields = ["density", "velocity", "velocity_divergence"]
+    out = api.sample(dm, rho, v, None, fields, spec, method=method, samples=5, seed=9, want_cells=True)
+    rng = np.random.default_rng(8)
+    up = {f: torch.from_numpy(rng.standard_normal(tuple(out[f].shape))).cuda() for f in fields}
+    g0, rb0 = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=method, samples=5, seed=9)
+    g1, rb1 = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=method, samples=5, seed=9,
+                          start_cells=out["located_cell"])
+    for name in ("position", "weight", "velocity"):
+        err = (g1[name] - g0[name]).abs().max().item() / g0[name].abs().max().item()
+        assert err < 1e-12, (name, err)
+    assert abs(rb1 - rb0) <= 1e-12 * abs(rb0)
