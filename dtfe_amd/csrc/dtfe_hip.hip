@@ -1147,11 +1147,16 @@ BrickGrid brick_grid(const int64_t n[3]) {
     return g;
 }
 
-// The row of this thread in brick order, -1 past the grid's edge.
-__device__ __forceinline__ int64_t brick_row(const SampleArgs &a, const BrickGrid &g) {
-    const int64_t nblk = gridDim.x;
-    const int64_t i = blockIdx.x, q = nblk / kXCDs, r = nblk % kXCDs, x = i % kXCDs;
-    const int64_t lb = x * q + min(x, r) + i / kXCDs;  // XCD x runs one consecutive range
+// Workgroup i's position in a launch of n: consecutive workgroups go to
+// different XCDs, so each XCD is given one consecutive range instead.
+__device__ __forceinline__ int64_t xcd_range(int64_t i, int64_t n) {
+    const int64_t q = n / kXCDs, r = n % kXCDs, x = i % kXCDs;
+    return x * q + min(x, r) + i / kXCDs;
+}
+
+// The row of this thread in brick lb (brick order: kTile^3 bricks per tile),
+// -1 past the grid's edge.
+__device__ __forceinline__ int64_t brick_row(const SampleArgs &a, const BrickGrid &g, int64_t lb) {
     const int64_t tile = lb / (kTile * kTile * kTile);
     const int w = (int)(lb % (kTile * kTile * kTile));
     const int64_t tk = tile % g.nt[2], tj = (tile / g.nt[2]) % g.nt[1], ti = tile / ((int64_t)g.nt[2] * g.nt[1]);
@@ -1169,7 +1174,8 @@ template <int KIND>
 __global__ void __launch_bounds__(kBlock) k_locate_centres(SampleArgs a, int32_t *__restrict__ centre, BrickGrid bg) {
     const bool bricks = DTFE_CENTRE_BRICKS && KIND != DTFE_SAMPLE_POINTS;
     int64_t stride = bricks ? a.n_rows : (int64_t)gridDim.x * blockDim.x;
-    int64_t row0 = bricks ? brick_row(a, bg) : (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t row0 = bricks ? brick_row(a, bg, xcd_range(blockIdx.x, gridDim.x))
+                          : (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row0 < 0) return;
     for (int64_t row = row0; row < a.n_rows; row += stride) {
         d3 c = row_centre<KIND>(a, row);
@@ -1345,6 +1351,10 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_plain(SampleArgs a) {
 #ifndef DTFE_ADJ_MIN_WAVES
 #define DTFE_ADJ_MIN_WAVES 2
 #endif
+// bricks per workgroup between two flushes of its LDS sums (more were slower:
+// fewer workgroups in flight to hide the walks)
+constexpr int kAdjBricks = 1;
+
 template <typename F, bool VEL, int KIND>
 __global__ void __launch_bounds__(kBlock, DTFE_ADJ_MIN_WAVES) k_adjoint_rows(SampleArgs a, BrickGrid bg) {
     __shared__ int keys[kSinkSlots];
@@ -1354,8 +1364,12 @@ __global__ void __launch_bounds__(kBlock, DTFE_ADJ_MIN_WAVES) k_adjoint_rows(Sam
     if (KIND != DTFE_SAMPLE_POINTS) {
         sink.clear();
         __syncthreads();
-        const int64_t row = brick_row(a, bg);
-        if (row >= 0 && !adjoint_row<F, VEL, false, NoWords, LdsSink, KIND>(a, row, nw, sink)) defer_row(a, row, nw);
+        const int64_t g0 = xcd_range(blockIdx.x, gridDim.x) * kAdjBricks;
+        for (int b = 0; b < kAdjBricks; ++b) {
+            const int64_t row = brick_row(a, bg, g0 + b);
+            if (row >= 0 && !adjoint_row<F, VEL, false, NoWords, LdsSink, KIND>(a, row, nw, sink))
+                defer_row(a, row, nw);
+        }
         __syncthreads();
         sink.template flush<F>(a);
         return;
@@ -1453,7 +1467,11 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_exact_all(SampleArgs a) {
 // 256 tetrahedra are first merged in an LDS hash table (Morton-ordered
 // meshes make them share ~4x fewer distinct vertices), then each distinct
 // vertex gets one FP64 atomic in HBM.
-constexpr int kAggSlots = 2048;  // > 4 * kBlock, load factor <= 0.5
+#ifndef DTFE_VOL_CHUNK
+#define DTFE_VOL_CHUNK 1
+#endif
+constexpr int kVolChunk = DTFE_VOL_CHUNK;     // tetrahedra per thread between flushes
+constexpr int kAggSlots = 2048 * kVolChunk;  // load factor <= 0.5
 
 __global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restrict__ pos,
                                                          const int4 *__restrict__ cells,
@@ -1461,14 +1479,16 @@ __global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restric
                                                          double *__restrict__ W, uint8_t *__restrict__ hull) {
     __shared__ int keys[kAggSlots];
     __shared__ double vals[kAggSlots];
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t span = (int64_t)blockDim.x * kVolChunk;
+    for (int64_t base = (int64_t)blockIdx.x * span; base < nc; base += (int64_t)gridDim.x * span) {
         for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
             keys[i] = -1;
             vals[i] = 0.0;
         }
         __syncthreads();
-        const int64_t t = base + threadIdx.x;
-        if (t < nc) {
+        for (int r = 0; r < kVolChunk; ++r) {
+            const int64_t t = base + (int64_t)r * blockDim.x + threadIdx.x;
+            if (t >= nc) continue;
             int4 c = cells[t];
             d3 p0 = load_pos(pos, c.x), p1 = load_pos(pos, c.y), p2 = load_pos(pos, c.z), p3 = load_pos(pos, c.w);
             d3 a = sub(p1, p0), b = sub(p2, p0), d = sub(p3, p0);
@@ -1478,14 +1498,14 @@ __global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restric
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int v = id[q];
-                unsigned slot = ((unsigned)v * 2654435761u) >> 21;  // 11 bits
+                unsigned slot = (unsigned)(((uint64_t)((unsigned)v * 2654435761u) * kAggSlots) >> 32);
                 for (int probe = 0; probe < kAggSlots; ++probe) {
                     int k = atomicCAS(&keys[slot], -1, v);
                     if (k == -1 || k == v) {
                         atomicAdd(&vals[slot], vol);
                         break;
                     }
-                    slot = (slot + 1) & (kAggSlots - 1);
+                    slot = slot + 1 == (unsigned)kAggSlots ? 0u : slot + 1;
                 }
             }
             int4 nb = nbrs[t];
@@ -1696,7 +1716,17 @@ __global__ void k_adj_vertex_rho(const F *__restrict__ w, const F *__restrict__ 
 // As in k_cell_volumes, a workgroup's 4 x 256 vertex contributions are first
 // merged in an LDS hash table (FP64), then each distinct vertex gets three
 // atomics in HBM.
-constexpr int kAdjSlots = 1024;  // >= 4 * kBlock keys fit
+// Tetrahedra per thread between two flushes: a workgroup merges kAdjChunk x
+// kBlock Morton-consecutive tetrahedra, whose vertices repeat more the
+// larger the chunk; a vertex that finds no slot goes straight to HBM.
+#ifndef DTFE_ADJ_CHUNK
+#define DTFE_ADJ_CHUNK 16
+#endif
+constexpr int kAdjChunk = DTFE_ADJ_CHUNK;
+#ifndef DTFE_ADJ_SLOTS
+#define DTFE_ADJ_SLOTS 2048
+#endif
+constexpr int kAdjSlots = DTFE_ADJ_SLOTS;
 
 template <typename F>
 __global__ void __launch_bounds__(kBlock) k_adj_tets(const double *__restrict__ pos, const int4 *__restrict__ cells,
@@ -1704,38 +1734,45 @@ __global__ void __launch_bounds__(kBlock) k_adj_tets(const double *__restrict__ 
                                                      F *__restrict__ d_pos) {
     __shared__ int keys[kAdjSlots];
     __shared__ double vals[kAdjSlots][3];
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t span = (int64_t)blockDim.x * kAdjChunk;
+    for (int64_t base = (int64_t)blockIdx.x * span; base < nc; base += (int64_t)gridDim.x * span) {
         for (int i = threadIdx.x; i < kAdjSlots; i += blockDim.x) {
             keys[i] = -1;
             vals[i][0] = vals[i][1] = vals[i][2] = 0.0;
         }
         __syncthreads();
-        const int64_t t = base + threadIdx.x;
-        if (t < nc) {
+        for (int r = 0; r < kAdjChunk; ++r) {
+            const int64_t t = base + (int64_t)r * blockDim.x + threadIdx.x;
+            if (t >= nc) continue;
             const int4 c = cells[t];
             const double dV = (dW[c.x] + dW[c.y]) + (dW[c.z] + dW[c.w]);
-            if (dV != 0.0) {
-                d3 x0 = load_pos(pos, c.x), x1 = load_pos(pos, c.y), x2 = load_pos(pos, c.z), x3 = load_pos(pos, c.w);
-                d3 e1 = sub(x1, x0), e2 = sub(x2, x0), e3 = sub(x3, x0);
-                d3 g1 = cross(e2, e3), g2 = cross(e3, e1), g3 = cross(e1, e2);
-                const double k = dV / 6.0;
-                d3 g0{-(g1.x + g2.x + g3.x), -(g1.y + g2.y + g3.y), -(g1.z + g2.z + g3.z)};
-                const int id[4] = {c.x, c.y, c.z, c.w};
-                const d3 gg[4] = {g0, g1, g2, g3};
+            if (dV == 0.0) continue;
+            d3 x0 = load_pos(pos, c.x), x1 = load_pos(pos, c.y), x2 = load_pos(pos, c.z), x3 = load_pos(pos, c.w);
+            d3 e1 = sub(x1, x0), e2 = sub(x2, x0), e3 = sub(x3, x0);
+            d3 g1 = cross(e2, e3), g2 = cross(e3, e1), g3 = cross(e1, e2);
+            const double k = dV / 6.0;
+            d3 g0{-(g1.x + g2.x + g3.x), -(g1.y + g2.y + g3.y), -(g1.z + g2.z + g3.z)};
+            const int id[4] = {c.x, c.y, c.z, c.w};
+            const d3 gg[4] = {g0, g1, g2, g3};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int v = id[q];
-                    unsigned slot = ((unsigned)v * 2654435761u) >> 22;  // 10 bits
-                    for (int probe = 0; probe < kAdjSlots; ++probe) {
-                        const int kk = atomicCAS(&keys[slot], -1, v);
-                        if (kk == -1 || kk == v) {
-                            atomicAdd(&vals[slot][0], k * gg[q].x);
-                            atomicAdd(&vals[slot][1], k * gg[q].y);
-                            atomicAdd(&vals[slot][2], k * gg[q].z);
-                            break;
-                        }
-                        slot = (slot + 1) & (kAdjSlots - 1);
+            for (int q = 0; q < 4; ++q) {
+                const int v = id[q];
+                unsigned slot = (unsigned)(((uint64_t)((unsigned)v * 2654435761u) * kAdjSlots) >> 32);
+                bool done = false;
+                for (int probe = 0; probe < 32 && !done; ++probe) {
+                    const int kk = atomicCAS(&keys[slot], -1, v);
+                    if (kk == -1 || kk == v) {
+                        atomicAdd(&vals[slot][0], k * gg[q].x);
+                        atomicAdd(&vals[slot][1], k * gg[q].y);
+                        atomicAdd(&vals[slot][2], k * gg[q].z);
+                        done = true;
                     }
+                    slot = slot + 1 == (unsigned)kAdjSlots ? 0u : slot + 1;
+                }
+                if (!done) {
+                    atomicAdd(d_pos + 3 * (int64_t)v + 0, (F)(k * gg[q].x));
+                    atomicAdd(d_pos + 3 * (int64_t)v + 1, (F)(k * gg[q].y));
+                    atomicAdd(d_pos + 3 * (int64_t)v + 2, (F)(k * gg[q].z));
                 }
             }
         }
@@ -1882,6 +1919,25 @@ __global__ void __launch_bounds__(kBlock) k_cell_records(const double *__restric
     }
 }
 
+// Zeroes up to four arrays in one launch.
+template <typename T>
+__global__ void k_zero4(T *p0, int64_t n0, T *p1, int64_t n1, T *p2, int64_t n2, T *p3, int64_t n3) {
+    const int64_t n = n0 + n1 + n2 + n3;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < n0) p0[i] = (T)0;
+        else if (i < n0 + n1) p1[i - n0] = (T)0;
+        else if (i < n0 + n1 + n2) p2[i - n0 - n1] = (T)0;
+        else p3[i - n0 - n1 - n2] = (T)0;
+    }
+}
+
+__global__ void k_diff(double *out, const double *a, const double *b,
+                       int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = a[i] - b[i];
+}
+
 template <typename T>
 __global__ void k_fill(T *p, int64_t n, T v) {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1910,7 +1966,7 @@ struct dtfe_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, plan, flags, rows, mass, mom,
-        defer_rows, defer_words, adj_rho, adj_dW, adj_partial, centre, records;
+        defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, centre, records;
     int last_deferred = 0;
     int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples
     // HIP events around the main kernels of the last calls (on ctx->stream)
@@ -2073,7 +2129,7 @@ void launch_sample(const SampleArgs &a, int which, int mc, int grid, size_t lds,
         hipLaunchKernelGGL((k_sample_mc<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), lds, s, a);
     else if (ADJ) {
         const BrickGrid bg = brick_grid(a.n);
-        const dim3 gb((unsigned)bg.blocks());
+        const dim3 gb((unsigned)(bg.blocks() / kAdjBricks));  // blocks() is a multiple of kTile^3
         switch (a.kind) {
             case DTFE_SAMPLE_GRID: hipLaunchKernelGGL((k_adjoint_rows<F, VEL, DTFE_SAMPLE_GRID>), gb, dim3(kBlock), 0, s, a, bg); break;
             case DTFE_SAMPLE_CONE: hipLaunchKernelGGL((k_adjoint_rows<F, VEL, DTFE_SAMPLE_CONE>), gb, dim3(kBlock), 0, s, a, bg); break;
@@ -2456,7 +2512,7 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_b, &ctx->bbox_part,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
-                      &ctx->adj_partial, &ctx->centre, &ctx->records};
+                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->centre, &ctx->records};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 9; ++i)
@@ -2526,7 +2582,7 @@ int dtfe_hip_vertex_density(dtfe_ctx *ctx, const dtfe_mesh *m, int dtype, const 
     hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (uint8_t *)ctx->hull.p, nv,
                        (uint8_t)0);
     HIP_TRY(hipEventRecord(ctx->ev[6], ctx->stream));
-    hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
+    hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for((nc + kVolChunk - 1) / kVolChunk)), dim3(kBlock), 0, ctx->stream, m->pos,
                        (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
                        (uint8_t *)ctx->hull.p);
     double factor = (3 + 1.0) / rho_bar;
@@ -2742,12 +2798,24 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         if (f64) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (double *)p, n, 0.0);
         else hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (float *)p, n, 0.0f);
     };
-    zero(gr->position, 3 * nv);
-    zero(gr->weight, nv);
-    zero(gr->velocity, 3 * nv);
-    if (nv < 4 || nc == 0) return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: gradients are zero");
+    if (nv < 4 || nc == 0) {
+        zero(gr->position, 3 * nv);
+        zero(gr->weight, nv);
+        zero(gr->velocity, 3 * nv);
+        return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: gradients are zero");
+    }
     if ((rc = ensure(ctx, ctx->adj_rho, nv * esz))) return rc;
-    zero(ctx->adj_rho.p, nv);
+    {
+        // every accumulator starts at zero (one launch)
+        const int64_t n0 = gr->position ? 3 * nv : 0, n1 = gr->weight ? nv : 0, n2 = gr->velocity ? 3 * nv : 0;
+        const int g = grid_for(n0 + n1 + n2 + nv);
+        if (f64)
+            hipLaunchKernelGGL(k_zero4<double>, dim3(g), dim3(kBlock), 0, ctx->stream, (double *)gr->position, n0,
+                               (double *)gr->weight, n1, (double *)gr->velocity, n2, (double *)ctx->adj_rho.p, nv);
+        else
+            hipLaunchKernelGGL(k_zero4<float>, dim3(g), dim3(kBlock), 0, ctx->stream, (float *)gr->position, n0,
+                               (float *)gr->weight, n1, (float *)gr->velocity, n2, (float *)ctx->adj_rho.p, nv);
+    }
 
     SampleArgs a{};
     a.pos = m->pos;
@@ -2795,46 +2863,59 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
                                (const float *)weight, (const float *)vf->density, (const float *)ctx->adj_rho.p,
                                nv, rho_bar, (float *)gr->weight, (double *)ctx->adj_dW.p,
                                (double *)ctx->adj_partial.p, need_W);
+        auto adj_tets = [&](const double *dW) {
+            if (!gr->position) return;
+            const dim3 g(grid_for((nc + kAdjChunk - 1) / kAdjChunk));
+            if (f64)
+                hipLaunchKernelGGL(k_adj_tets<double>, g, dim3(kBlock), 0, ctx->stream, m->pos, (const int4 *)m->cells,
+                                   nc, dW, (double *)gr->position);
+            else
+                hipLaunchKernelGGL(k_adj_tets<float>, g, dim3(kBlock), 0, ctx->stream, m->pos, (const int4 *)m->cells,
+                                   nc, dW, (float *)gr->position);
+        };
+        adj_tets((const double *)ctx->adj_dW.p);
+        std::vector<double> part(parts + 1);
+        auto read_partials = [&]() -> int {
+            HIP_TRY(hipMemcpyAsync(part.data(), ctx->adj_partial.p, parts * sizeof(double) + sizeof(int),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            return DTFE_OK;
+        };
+        if ((rc = read_partials())) return rc;
         int need = 0;
-        HIP_TRY(hipMemcpyAsync(&need, need_W, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(&need, part.data() + parts, sizeof(int));
         if (need) {
-            // a zero-weight interior vertex: the incident volumes from the mesh
+            // a zero-weight interior vertex needs the incident volumes: redo the
+            // chain from the mesh and add the change of dL/dW to dL/dX (the
+            // volume term is linear in dL/dW)
             if ((rc = ensure(ctx, ctx->W, nv * sizeof(double)))) return rc;
             if ((rc = ensure(ctx, ctx->hull, nv))) return rc;
+            if ((rc = ensure(ctx, ctx->adj_dW2, nv * sizeof(double)))) return rc;
             if (gr->weight) zero(gr->weight, nv);
             hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->W.p,
                                nv, 0.0);
             hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
                                (uint8_t *)ctx->hull.p, nv, (uint8_t)0);
-            hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
-                               (const int4 *)m->cells, (const int4 *)m->neighbors, nc, (double *)ctx->W.p,
-                               (uint8_t *)ctx->hull.p);
+            hipLaunchKernelGGL(k_cell_volumes, dim3(grid_for((nc + kVolChunk - 1) / kVolChunk)), dim3(kBlock), 0,
+                               ctx->stream, m->pos, (const int4 *)m->cells, (const int4 *)m->neighbors, nc,
+                               (double *)ctx->W.p, (uint8_t *)ctx->hull.p);
             if (f64)
                 hipLaunchKernelGGL(k_adj_vertex<double>, dim3(parts), dim3(kBlock), 0, ctx->stream,
                                    (const double *)weight, (const double *)vf->density, (const double *)ctx->W.p,
                                    (const uint8_t *)ctx->hull.p, (const double *)ctx->adj_rho.p, nv, rho_bar,
-                                   (double *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+                                   (double *)gr->weight, (double *)ctx->adj_dW2.p, (double *)ctx->adj_partial.p);
             else
                 hipLaunchKernelGGL(k_adj_vertex<float>, dim3(parts), dim3(kBlock), 0, ctx->stream,
                                    (const float *)weight, (const float *)vf->density, (const double *)ctx->W.p,
                                    (const uint8_t *)ctx->hull.p, (const float *)ctx->adj_rho.p, nv, rho_bar,
-                                   (float *)gr->weight, (double *)ctx->adj_dW.p, (double *)ctx->adj_partial.p);
+                                   (float *)gr->weight, (double *)ctx->adj_dW2.p, (double *)ctx->adj_partial.p);
+            hipLaunchKernelGGL(k_diff, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->adj_dW.p,
+                               (const double *)ctx->adj_dW2.p, (const double *)ctx->adj_dW.p, nv);
+            adj_tets((const double *)ctx->adj_dW.p);
+            if ((rc = read_partials())) return rc;
         }
-        if (gr->position) {
-            if (f64)
-                hipLaunchKernelGGL(k_adj_tets<double>, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
-                                   (const int4 *)m->cells, nc, (const double *)ctx->adj_dW.p, (double *)gr->position);
-            else
-                hipLaunchKernelGGL(k_adj_tets<float>, dim3(grid_for(nc)), dim3(kBlock), 0, ctx->stream, m->pos,
-                                   (const int4 *)m->cells, nc, (const double *)ctx->adj_dW.p, (float *)gr->position);
-        }
-        std::vector<double> part(parts);
-        HIP_TRY(hipMemcpyAsync(part.data(), ctx->adj_partial.p, parts * sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
         double sum = 0.0;
-        for (double x : part) sum += x;
+        for (int i = 0; i < parts; ++i) sum += part[i];
         gr->rho_bar = sum;
     }
     HIP_TRY(hipGetLastError());
@@ -2845,8 +2926,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
 int dtfe_hip_last_deferred(dtfe_ctx *ctx) { return ctx ? ctx->last_deferred : -1; }
 
 // Kernel times of the last calls, measured with HIP events on the context's
-// stream (synchronises it): ms[0] main sampling kernel, ms[1] exact-sign
-// kernel, ms[2] seed-grid build, ms[3] vertex density.
+// stream (synchronises it); the order of ms[] is the header's.
 int dtfe_hip_last_timings(dtfe_ctx *ctx, double *ms) {
     if (!ctx || !ms) return DTFE_ERR_ARGUMENT;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
