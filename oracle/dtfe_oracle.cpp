@@ -377,7 +377,10 @@ int64_t oracle_locate(int64_t nv, const double *pos, int64_t nc, const int32_t *
 // Carlo average, 3 = equidistant average (grid only). arg_order: order in
 // which the three draws of one grid sample are evaluated (0 = x,y,z as
 // clang does; 1 = z,y,x). Returns the number of output rows (user points
-// outside the region produce no row), or -1 if a walk failed.
+// outside the region produce no row), or -1 if a walk failed. o_points
+// (optional, [rows][NN][3]) and o_point_cell ([rows][NN]) receive every sample
+// point the averages are taken over and its located cell (-1 outside the
+// hull); with method 0 the single point is the row's centre.
 int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *cells,
                       const int32_t *nbrs, const double *rho, const double *vel, const double *scalar,
                       int S, int kind, const double *box, const int64_t *n, const double *cone,
@@ -385,7 +388,7 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
                       int method, int NN, uint64_t seed, int partNo, int arg_order, int value_bug,
                       const int *fields, double *o_density, double *o_velocity, double *o_velgrad,
                       double *o_velstd, double *o_scalar, double *o_scalargrad,
-                      int64_t *o_cell_of_centre) {
+                      int64_t *o_cell_of_centre, double *o_points, int64_t *o_point_cell) {
     Mesh m{nv, pos, nc, cells, nbrs};
     VertexData vd{rho, vel, scalar, S};
     FieldReq f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
@@ -405,6 +408,12 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
         if (t == -2) failed = true;
         if (t >= 0) prevc = t;
         return t;
+    };
+    const int slots = NN > 0 ? NN : 1;
+    auto keep_point = [&](int64_t row, int w, const double p[3], int64_t t) {
+        if (!o_points) return;
+        for (int a = 0; a < 3; ++a) o_points[((size_t)row * slots + w) * 3 + a] = p[a];
+        o_point_cell[(size_t)row * slots + w] = t;
     };
 
     if (kind == 0 || kind == 1) {
@@ -457,6 +466,7 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
                     if (method == 0) {
                         if (centre >= 0) evaluate(m, vd, f, value_bug, centre, c, acc, nullptr);
                         store(f, S, acc, 1.0, o, out);
+                        keep_point(o, 0, c, centre);
                         z += dx[2];
                         continue;
                     }
@@ -478,6 +488,7 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
                             p[2] = origin[2] + tr * std::cos(tt);
                         }
                         int64_t t = loc(p, centre >= 0 ? centre : -1);
+                        keep_point(o, w, p, t);
                         double *vv = &velsamples[3 * w];
                         vv[0] = vv[1] = vv[2] = 0.;
                         if (t < 0) continue;
@@ -511,6 +522,7 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
         if (method == 0) {
             if (centre >= 0) evaluate(m, vd, f, value_bug, centre, c, acc, nullptr);
             store(f, S, acc, 1.0, o, out);
+            keep_point(o, 0, c, centre);
             ++o;
             continue;
         }
@@ -521,6 +533,7 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
             p[1] = q[1] + uni() * dl[1];
             p[2] = q[2] + uni() * dl[2];
             int64_t t = loc(p, centre >= 0 ? centre : -1);
+            keep_point(o, w, p, t);
             if (t < 0) continue;
             evaluate(m, vd, f, value_bug, t, p, acc, nullptr);
         }

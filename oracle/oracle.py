@@ -42,7 +42,7 @@ def load():
     lib.oracle_locate.argtypes = [i64, P, i64, P, P, i64, P, P]
     lib.oracle_locate.restype = i64
     lib.oracle_sample.argtypes = [i64, P, i64, P, P, P, P, P, i32, i32, P, P, P, P, i64, P, P,
-                                  i32, i32, C.c_uint64, i32, i32, i32, P, P, P, P, P, P, P, P]
+                                  i32, i32, C.c_uint64, i32, i32, i32, P, P, P, P, P, P, P, P, P, P]
     lib.oracle_sample.restype = i64
     lib.oracle_div_shear_vort.argtypes = [i64, P, P, P, P]
     lib.oracle_deposit.argtypes = [i32, i64, P, P, P, P, P, d, i32, i32, P, P]
@@ -117,8 +117,11 @@ def locate(mesh, pts):
 
 def sample(mesh, rho, vel=None, scalar=None, fields=("density",), kind="grid", box=None, n=None,
            cone=None, origin=None, points=None, deltas=None, method=0, samples=1, seed=0, part_no=-1,
-           arg_order=0, value_bug=True):
-    """Runs the oracle's sampling driver; returns a dict of float64 arrays."""
+           arg_order=0, value_bug=True, want_points=False):
+    """Runs the oracle's sampling driver; returns a dict of float64 arrays.
+    want_points adds 'points' [rows, M, 3] (every sample point, the row's
+    centre when method is 0) and 'point_cell' [rows, M] (its located cell,
+    -1 outside the hull)."""
     lib = load()
     pos, cells, nbrs = _f64(mesh.pos), _i32(mesh.cells), _i32(mesh.nbrs)
     rho, vel = _f64(rho), _f64(vel)
@@ -147,14 +150,20 @@ def sample(mesh, rho, vel=None, scalar=None, fields=("density",), kind="grid", b
     for f in FIELDS:
         outs[f] = np.zeros((n_rows, comp[f]) if comp[f] != 1 else n_rows) if want[f] else None
     cells_out = np.zeros(n_rows, dtype=np.int64)
+    slots = 1 if method == 0 else int(samples)
+    pts_out = np.zeros((n_rows, slots, 3)) if want_points else None
+    pcell_out = np.zeros((n_rows, slots), dtype=np.int64) if want_points else None
     nw = lib.oracle_sample(pos.shape[0], _p(pos), cells.shape[0], _p(cells), _p(nbrs), _p(rho), _p(vel),
                            _p(scalar), S, k, _p(bx), _p(nn), _p(cn), _p(og), npts, _p(pts), _p(dl),
                            method, samples, seed, part_no, arg_order, int(value_bug), _p(flags),
-                           *[_p(outs[f]) for f in FIELDS], _p(cells_out))
+                           *[_p(outs[f]) for f in FIELDS], _p(cells_out), _p(pts_out), _p(pcell_out))
     if nw < 0:
         raise RuntimeError("oracle sampling failed (walk did not terminate or bad arguments)")
     res = {f: outs[f][:nw] for f in FIELDS if want[f]}
     res["located_cell"] = cells_out[:nw]
+    if want_points:
+        res["points"] = pts_out[:nw]
+        res["point_cell"] = pcell_out[:nw]
     grad_needed = [f for f in ("velocity_divergence", "velocity_shear", "velocity_vorticity") if f in fields]
     if grad_needed:
         if "velocity_gradient" not in res:

@@ -178,3 +178,71 @@ def test_adjoint_start_cells(gpu, method):
         err = (g1[name] - g0[name]).abs().max().item() / g0[name].abs().max().item()
         assert err < 1e-12, (name, err)
     assert abs(rb1 - rb0) <= 1e-12 * abs(rb0)
+
+
+@pytest.mark.parametrize("kind", ["grid", "cone", "points"])
+def test_adjoint_monte_carlo_matches_fp64_autograd(gpu, kind):
+    """Cell-averaged (Monte-Carlo) rows: the reference averages the FP64
+    tensor forward over the oracle's sample points of each row (a sample
+    outside the hull adds zero and still counts in the divisor), then autograd
+    gives dL/d(position, weight, velocity, rho_bar)."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    m = prob.mesh
+    M_ = 4
+    dm, w, v, rho = _setup(prob, torch.float64)
+    fields = ["density", "velocity"]
+    if kind == "grid":
+        n = (7, 6, 5)
+        spec = api.GridSpec(prob.region, n)
+        okw = dict(kind="grid", box=prob.region, n=n)
+        fields += ["velocity_gradient", "velocity_divergence"]
+    elif kind == "cone":
+        cone, n, origin = (1.0, 17.0, 15.0, 75.0, 15.0, 75.0), (12, 6, 6), (1.0, 1.0, 1.0)
+        spec = api.ConeSpec(cone, n, origin)
+        okw = dict(kind="cone", cone=cone, n=n, origin=origin)
+    else:
+        rng = np.random.default_rng(12)
+        pts = rng.random((400, 3)) * 22.0 - 1.0          # a few outside the region box
+        deltas = rng.uniform(0.2, 1.0, pts.shape)
+        box = (0.0, 20.0, 0.0, 20.0, 0.0, 20.0)
+        spec = api.PointsSpec(torch.from_numpy(pts).cuda(), torch.from_numpy(deltas).cuda(), box)
+        okw = dict(kind="points", box=box, points=pts, deltas=deltas)
+    out = api.sample(dm, rho, v, None, fields, spec, method=2, samples=M_, seed=31)
+    rho_o, _, hull = O.vertex_density(m, prob.w, prob.rho_bar)
+    ref = O.sample(m, rho_o, prob.vel, None, fields=["density", "velocity", "velocity_gradient"], method=2,
+                   samples=M_, seed=31, want_points=True, **okw)
+    n_rows = out["density"].shape[0]
+    assert ref["density"].shape[0] == n_rows
+    assert np.abs(out["density"].cpu().numpy() - ref["density"]).max() <= 1e-9 * np.abs(ref["density"]).max()
+    pc = ref["point_cell"].reshape(-1)
+    finite = pc >= 0
+    assert finite.mean() > 0.5
+    row_of = torch.from_numpy(np.repeat(np.arange(n_rows), M_)[finite])
+    rng = np.random.default_rng(13)
+    up = {f: torch.from_numpy(rng.standard_normal(tuple(out[f].shape))).cuda() for f in fields}
+    g, g_rb = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=2, samples=M_, seed=31)
+
+    pos = torch.from_numpy(m.pos).clone().requires_grad_(True)
+    wt = torch.from_numpy(prob.w).clone().requires_grad_(True)
+    vt = torch.from_numpy(prob.vel).clone().requires_grad_(True)
+    rb = torch.tensor(prob.rho_bar, dtype=torch.float64, requires_grad=True)
+    d, vel, G = R.forward(pos, wt, vt, torch.from_numpy(m.cells).long(), torch.from_numpy(hull),
+                          torch.from_numpy(pc[finite]), torch.from_numpy(ref["points"].reshape(-1, 3)[finite]),
+                          rb, True)
+
+    def rows(x):
+        shape = (n_rows,) + tuple(x.shape[1:])
+        return torch.zeros(shape, dtype=x.dtype).index_add(0, row_of, x) / M_
+
+    per_row = {"density": rows(d), "velocity": rows(vel), "velocity_gradient": rows(G)}
+    Gr = per_row["velocity_gradient"]
+    per_row["velocity_divergence"] = Gr[:, 0] + Gr[:, 4] + Gr[:, 8]
+    L = sum((per_row[f] * up[f].cpu()).sum() for f in fields)
+    L.backward()
+    for name, refg in (("position", pos.grad), ("weight", wt.grad), ("velocity", vt.grad)):
+        err = (g[name].cpu() - refg).abs().max().item() / refg.abs().max().item()
+        print(f"monte-carlo adjoint {kind} {name}: max rel err {err:.3e}")
+        assert err < 1e-9, name
+    err_rb = abs(g_rb - rb.grad.item()) / abs(rb.grad.item())
+    assert err_rb < 1e-9
