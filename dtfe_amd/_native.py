@@ -50,6 +50,7 @@ EXPORTED = [
     "dtfe_hip_last_error", "dtfe_hip_vertex_density", "dtfe_hip_sample", "dtfe_hip_deposit",
     "dtfe_hip_version", "dtfe_hip_last_timings", "dtfe_hip_last_deferred", "dtfe_hip_walk_failures",
     "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint", "dtfe_hip_locator_bytes", "dtfe_hip_build_locator",
+    "dtfe_hip_periodic_images", "dtfe_hip_average_density", "dtfe_hip_gather_rows", "dtfe_hip_scatter_add_rows",
 ]
 
 CELL_RECORD_BYTES = 128
@@ -135,6 +136,11 @@ def load(path: str = LIB_PATH):
         lib.dtfe_hip_walk_failures.argtypes = [vp, C.POINTER(C.c_int)]
         lib.dtfe_hip_last_timings.argtypes = [vp, C.POINTER(C.c_double * 5)]
         lib.dtfe_hip_last_walk_stats.argtypes = [vp, C.POINTER(C.c_int64 * 4)]
+        lib.dtfe_hip_periodic_images.argtypes = [vp, i64, i32, vp, C.POINTER(C.c_double * 6),
+                                                 C.POINTER(C.c_double * 6), i32, i64, vp, vp, C.POINTER(i64)]
+        lib.dtfe_hip_average_density.argtypes = [vp, i64, i32, vp, C.POINTER(C.c_double)]
+        lib.dtfe_hip_gather_rows.argtypes = [vp, i64, vp, i64, i32, i32, vp, vp]
+        lib.dtfe_hip_scatter_add_rows.argtypes = [vp, i64, vp, i64, i32, i32, vp, vp]
         lib.dtfe_hip_adjoint.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), vp, C.c_double,
                                          C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Upstream),
                                          C.POINTER(VertexGrads)]

@@ -317,3 +317,104 @@ def adjoint(mesh: DeviceMesh, rho: torch.Tensor, weight: torch.Tensor, vel: Opti
                                   C.byref(mc), N.COMPAT_VALUE_BUG if value_bug else 0, C.byref(up), C.byref(gr))
     ctx.check(rc, allow=(N.DTFE_ERR_TOO_FEW_VERTICES,))
     return out, float(gr.rho_bar)
+
+
+# ---- host front end on the device -------------------------------------------------------------
+def periodic_images(pos: torch.Tensor, full_box, padded_box, periodic: bool = True):
+    """findParticlesInBox (src/subpartition.h:79-154) on the device.
+
+    pos [n, 3] float32 or float64 (the type is the arithmetic's, as Real in
+    the reference). Returns (image_pos [k, 3] of pos's dtype, image_of [k]
+    int32): for each of the 27 offsets in the reference's order, the
+    particles in input order whose position lies in padded_box translated by
+    the offset, shifted back by it; non-periodic: the particles inside
+    padded_box."""
+    ctx = context(pos.device)
+    pos = pos.contiguous()
+    code = _dtype_code(pos.dtype)
+    fb = (C.c_double * 6)(*[float(x) for x in full_box])
+    pb = (C.c_double * 6)(*[float(x) for x in padded_box])
+    count = C.c_int64()
+    n = int(pos.shape[0])
+    ctx.check(ctx.lib.dtfe_hip_periodic_images(ctx.h, n, code, _ptr(pos), C.byref(fb), C.byref(pb), int(periodic),
+                                               0, None, None, C.byref(count)))
+    k = int(count.value)
+    image_pos = torch.empty((k, 3), dtype=pos.dtype, device=pos.device)
+    image_of = torch.empty(k, dtype=torch.int32, device=pos.device)
+    if k:
+        ctx.check(ctx.lib.dtfe_hip_periodic_images(ctx.h, n, code, _ptr(pos), C.byref(fb), C.byref(pb),
+                                                   int(periodic), k, _ptr(image_pos), _ptr(image_of),
+                                                   C.byref(count)))
+    return image_pos, image_of
+
+
+def average_density(weight: torch.Tensor) -> float:
+    """averageDensity (src/subpartition.h:437-449) on the device: sum(w) / 35000^3."""
+    ctx = context(weight.device)
+    w = weight.contiguous()
+    out = C.c_double()
+    ctx.check(ctx.lib.dtfe_hip_average_density(ctx.h, int(w.numel()), _dtype_code(w.dtype), _ptr(w), C.byref(out)))
+    return float(out.value)
+
+
+def _rows(t: torch.Tensor):
+    t = t.contiguous()
+    return t, 1 if t.dim() == 1 else int(np.prod(t.shape[1:]))
+
+
+def gather_rows(src: torch.Tensor, index: torch.Tensor) -> torch.Tensor:
+    """src[index] through the library: per-vertex copies of particle data."""
+    ctx = context(src.device)
+    src, width = _rows(src)
+    idx = index.to(torch.int32).contiguous()
+    dst = torch.empty((idx.shape[0],) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    ctx.check(ctx.lib.dtfe_hip_gather_rows(ctx.h, int(idx.shape[0]), _ptr(idx), int(src.shape[0]),
+                                           _dtype_code(src.dtype), width, _ptr(src), _ptr(dst)))
+    return dst
+
+
+def scatter_add_rows(src: torch.Tensor, index: torch.Tensor, n_dst: int) -> torch.Tensor:
+    """out[index[i]] += src[i] into a zeroed [n_dst, ...] tensor: vertex
+    gradients summed onto their particles."""
+    ctx = context(src.device)
+    src, width = _rows(src)
+    idx = index.to(torch.int32).contiguous()
+    dst = torch.zeros((int(n_dst),) + tuple(src.shape[1:]), dtype=src.dtype, device=src.device)
+    ctx.check(ctx.lib.dtfe_hip_scatter_add_rows(ctx.h, int(idx.shape[0]), _ptr(idx), int(n_dst),
+                                                _dtype_code(src.dtype), width, _ptr(src), _ptr(dst)))
+    return dst
+
+
+@dataclass
+class DeviceParticles:
+    """Particles after DTFE()'s steps before the tessellation, on the device."""
+    pos: torch.Tensor        # [k, 3] image positions (the points to tessellate)
+    image_of: torch.Tensor   # [k] int32 particle index of every image
+    rho_bar: float
+    region: tuple
+    padding: float
+
+
+def prepare_particles(pos: torch.Tensor, weight: torch.Tensor, box, periodic: bool = True,
+                      padding_particles: float = 5.0, rho_bar: Optional[float] = None) -> DeviceParticles:
+    """Average density and periodic padding as DTFE() does them before its
+    tessellation (src/DTFE.cpp:101-103, :222-237), on the device; the host
+    mirror is dtfe_amd.mesh.prepare_particles."""
+    from .mesh import padding_length
+    box = tuple(float(x) for x in box)
+    if rho_bar is None:
+        rho_bar = average_density(weight)
+    pad = padding_length(int(pos.shape[0]), box[1] - box[0], padding_particles)
+    if periodic:
+        padded = tuple(box[i] - pad if i % 2 == 0 else box[i] + pad for i in range(6))
+        ipos, iof = periodic_images(pos, box, padded, periodic=True)
+    else:
+        ipos, iof = pos.contiguous(), torch.arange(pos.shape[0], dtype=torch.int32, device=pos.device)
+    return DeviceParticles(pos=ipos, image_of=iof, rho_bar=float(rho_bar), region=box, padding=pad)
+
+
+def vertex_particles(prep: DeviceParticles, mesh: Mesh) -> torch.Tensor:
+    """Particle index of every mesh vertex (int32, device): the image the
+    vertex was made from, mapped to its particle."""
+    pov = torch.from_numpy(np.ascontiguousarray(mesh.point_of_vertex)).to(prep.image_of.device, torch.long)
+    return prep.image_of[pov].contiguous()

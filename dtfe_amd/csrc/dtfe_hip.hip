@@ -1951,72 +1951,9 @@ __global__ void k_fill(T *p, int64_t n, T v) {
 // ============================================================================
 using namespace dtfe;
 
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-}  // namespace
-
-struct dtfe_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    std::string err;
-    DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, plan, flags, rows, mass, mom,
-        defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, centre, records;
-    int last_deferred = 0;
-    int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples
-    // HIP events around the main kernels of the last calls (on ctx->stream)
-    hipEvent_t ev[9] = {};
-    bool ev_pending[4] = {false, false, false, false};
-    // 0 sample main kernel, 1 exact kernel, 2 seed grid, 3 vertex density, 4 centre location
-    double last_ms[5] = {0, 0, 0, 0, 0};
-    // MC plan cache
-    bool plan_valid = false;
-    uint32_t plan_seed = 0;
-    int64_t plan_wpr = 0, plan_rows = 0, plan_rpb = 0;
-    std::vector<uint32_t> plan_host;
-    double last_plan_ms = 0.0;
-};
+#include "context.h"
 
 namespace {
-
-int fail(dtfe_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(ctx, DTFE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-int ensure(dtfe_ctx *ctx, DevBuf &b, size_t bytes) {
-    if (b.bytes >= bytes && b.p) return DTFE_OK;
-    if (b.p) {
-        hipError_t e = hipFree(b.p);
-        (void)e;
-        b.p = nullptr;
-        b.bytes = 0;
-    }
-    size_t sz = std::max<size_t>(bytes, 256);
-    if (hipMalloc(&b.p, sz) != hipSuccess) {
-        b.p = nullptr;
-        return fail(ctx, DTFE_ERR_NO_MEMORY, "device allocation of " + std::to_string(sz) + " bytes failed");
-    }
-    b.bytes = sz;
-    return DTFE_OK;
-}
-
-int grid_for(int64_t n, int block = kBlock) {
-    int64_t g = (n + block - 1) / block;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
-}
 
 int check_mesh(dtfe_ctx *ctx, const dtfe_mesh *m) {
     if (!m) return fail(ctx, DTFE_ERR_ARGUMENT, "mesh is NULL");
@@ -2512,7 +2449,7 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_b, &ctx->bbox_part,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
-                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->centre, &ctx->records};
+                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->centre, &ctx->records, &ctx->front};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 9; ++i)

@@ -212,6 +212,48 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *mesh, const dtfe_vertex_fie
                      double rho_bar, const dtfe_sampling *sampling, const dtfe_mc *mc, unsigned flags,
                      const dtfe_upstream *upstream, dtfe_vertex_grads *grads);
 
+/* ---- host front end on the device ------------------------------------------------
+ * Periodic images, replacing findParticlesInBox / insertParticlesInBox
+ * (src/subpartition.h:79-154). For each of the 27 offsets (i1, i2, i3 in
+ * 0..2, in that nesting order; offset = ((i1-1) Lx, (i2-1) Ly, (i3-1) Lz)
+ * with L = full_box's side lengths), every particle, in input order, whose
+ * position lies in padded_box translated by the offset (closed bounds) gives
+ * one image at position - offset. Non-periodic (periodic = 0): the particles
+ * inside padded_box, unshifted. dtype (DTFE_F32 / DTFE_F64) is the type of
+ * pos and image_pos and of the arithmetic, as Real is in the reference.
+ * pos [n][3] device. *n_images receives the number of images. With
+ * image_pos and image_of both NULL only the count is made; otherwise
+ * capacity must be at least the count (else nothing is written and
+ * DTFE_ERR_ARGUMENT is returned), image_pos is [capacity][3] and image_of
+ * [capacity] receives the particle index of each image. Synchronises the
+ * context's stream. */
+int dtfe_hip_periodic_images(dtfe_ctx *ctx, int64_t n, int dtype, const void *pos, const double full_box[6],
+                             const double padded_box[6], int periodic, int64_t capacity, void *image_pos,
+                             int32_t *image_of, int64_t *n_images);
+
+/* Average density, replacing averageDensity (src/subpartition.h:437-449):
+ * sum(weight) in FP64 divided by the reference's box volume 35000^3. The sum
+ * is a fixed-order tree, so it is the same on every call for a given n, and
+ * differs from the reference's serial loop only by rounding. Synchronises. */
+int dtfe_hip_average_density(dtfe_ctx *ctx, int64_t n, int dtype, const void *weight, double *rho_bar);
+
+/* dst[i][c] = src[index[i]][c] for i < n_rows, c < width (device arrays of
+ * `dtype`): the per-vertex copies of particle data that the reference makes
+ * when it inserts a particle into the triangulation
+ * (src/CGAL_triangulation/triangulation.cpp:176-185). A row whose index lies
+ * outside [0, n_src) is zero-filled and the call returns DTFE_ERR_ARGUMENT.
+ * Synchronises. */
+int dtfe_hip_gather_rows(dtfe_ctx *ctx, int64_t n_rows, const int32_t *index, int64_t n_src, int dtype,
+                         int width, const void *src, void *dst);
+
+/* dst[index[i]][c] += src[i][c] with atomic adds (the order of the additions,
+ * and so the last bits of dst, can change from call to call): the adjoint's
+ * vertex gradients summed onto their particles. No counterpart in the
+ * reference. dst is not cleared. Indices outside [0, n_dst) are skipped and
+ * the call returns DTFE_ERR_ARGUMENT. Synchronises. */
+int dtfe_hip_scatter_add_rows(dtfe_ctx *ctx, int64_t n_rows, const int32_t *index, int64_t n_dst, int dtype,
+                              int width, const void *src, void *dst);
+
 /* Library version string. */
 const char *dtfe_hip_version(void);
 

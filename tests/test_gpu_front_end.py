@@ -1,0 +1,167 @@
+"""The front end of DTFE() on the device: periodic images bit-exact against
+the oracle (positions, particle indices and their order), the average
+density against the oracle's serial sum, and the row gathers / scatters
+against torch indexing."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from dtfe_amd import mesh as M, synthetic as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _particles(n, L, seed):
+    """Clustered particles plus particles exactly on the padded-box faces,
+    on the box faces and in the corners (closed bounds decide them)."""
+    ps = S.clustered(n, L, 20, 0.2, 1.0, seed)
+    pad = M.padding_length(n, L)
+    edge = np.array([0.0, pad, L - pad, L, np.nextafter(L - pad, 0), np.nextafter(pad, L)])
+    rng = np.random.default_rng(seed)
+    extra = rng.choice(edge, size=(600, 3))
+    extra[:200, 1:] = rng.random((200, 2)) * L
+    pos = np.concatenate([ps.pos, extra])
+    return pos, pad
+
+
+@pytest.mark.parametrize("n,L", [(5000, 20.0), (40000, 50.0)])
+def test_periodic_images_bit_exact_fp64(gpu, n, L):
+    from dtfe_amd import api
+    pos, pad = _particles(n, L, 5)
+    box = (0.0, L, 0.0, L, 0.0, L)
+    padded = (-pad, L + pad, -pad, L + pad, -pad, L + pad)
+    ip, iof = api.periodic_images(torch.from_numpy(pos).cuda(), box, padded)
+    rp, rof = O.periodic_images(pos, box, padded)
+    print(f"images: {ip.shape[0]} of {pos.shape[0]} particles")
+    assert ip.shape[0] == rp.shape[0]
+    assert np.array_equal(iof.cpu().numpy().astype(np.int64), rof)
+    assert np.array_equal(ip.cpu().numpy().view(np.uint64), rp.view(np.uint64))
+
+
+def test_periodic_images_fp32_arithmetic(gpu):
+    """Real = float: the translated boxes and the shifted positions are
+    formed in single precision, as the reference's float build does."""
+    from dtfe_amd import api
+    pos, pad = _particles(5000, 20.0, 6)
+    pos32 = pos.astype(np.float32)
+    L = np.float32(20.0)
+    box = (0.0, 20.0, 0.0, 20.0, 0.0, 20.0)
+    pad32 = float(np.float32(pad))
+    padded = (-pad32, 20.0 + pad32, -pad32, 20.0 + pad32, -pad32, 20.0 + pad32)
+    ip, iof = api.periodic_images(torch.from_numpy(pos32).cuda(), box, padded)
+    outs, idxs = [], []
+    pb = np.asarray(padded, dtype=np.float32)
+    for t in range(27):
+        i = np.array([t // 9, (t // 3) % 3, t % 3])
+        off = (i - 1).astype(np.float32) * L
+        lo, hi = pb[0::2] + off, pb[1::2] + off
+        sel = np.nonzero(np.all((pos32 >= lo) & (pos32 <= hi), axis=1))[0]
+        outs.append(pos32[sel] - off)
+        idxs.append(sel)
+    exp_pos, exp_of = np.concatenate(outs), np.concatenate(idxs)
+    assert exp_pos.dtype == np.float32
+    assert np.array_equal(iof.cpu().numpy(), exp_of)
+    assert np.array_equal(ip.cpu().numpy().view(np.uint32), exp_pos.view(np.uint32))
+
+
+def test_non_periodic_selection(gpu):
+    from dtfe_amd import api
+    pos, _ = _particles(5000, 20.0, 7)
+    box = (0.0, 20.0, 0.0, 20.0, 0.0, 20.0)
+    sub = (2.0, 15.0, 0.0, 20.0, 5.0, 20.0)
+    ip, iof = api.periodic_images(torch.from_numpy(pos).cuda(), box, sub, periodic=False)
+    lo, hi = np.asarray(sub[0::2]), np.asarray(sub[1::2])
+    exp = np.nonzero(np.all((pos >= lo) & (pos <= hi), axis=1))[0]
+    assert np.array_equal(iof.cpu().numpy(), exp)
+    assert np.array_equal(ip.cpu().numpy(), pos[exp])
+
+
+def test_image_capacity_and_empty_input(gpu):
+    import ctypes as C
+    from dtfe_amd import _native as N, api
+    pos, pad = _particles(3000, 20.0, 8)
+    t = torch.from_numpy(pos).cuda()
+    box = (0.0, 20.0, 0.0, 20.0, 0.0, 20.0)
+    padded = (-pad, 20 + pad, -pad, 20 + pad, -pad, 20 + pad)
+    ctx = api.context(t.device)
+    fb, pb = (C.c_double * 6)(*box), (C.c_double * 6)(*padded)
+    cnt = C.c_int64()
+    small_pos = torch.empty((10, 3), dtype=torch.float64, device="cuda")
+    small_of = torch.empty(10, dtype=torch.int32, device="cuda")
+    rc = ctx.lib.dtfe_hip_periodic_images(ctx.h, t.shape[0], N.DTFE_F64, t.data_ptr(), C.byref(fb), C.byref(pb), 1,
+                                          10, small_pos.data_ptr(), small_of.data_ptr(), C.byref(cnt))
+    assert rc == N.DTFE_ERR_ARGUMENT and cnt.value > 10
+    ip, iof = api.periodic_images(torch.empty((0, 3), dtype=torch.float64, device="cuda"), box, padded)
+    assert ip.shape == (0, 3) and iof.shape == (0,)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_average_density(gpu, dtype):
+    from dtfe_amd import api
+    rng = np.random.default_rng(3)
+    for n in (1, 1000, 300001):
+        w = rng.lognormal(0.0, 0.3, n).astype(np.float64 if dtype == torch.float64 else np.float32)
+        got = api.average_density(torch.from_numpy(w).cuda())
+        exp = O.average_density(w.astype(np.float64))
+        # fixed-order tree against a serial loop: rounding only, bounded by n ulps
+        assert abs(got - exp) <= 4 * n * np.finfo(np.float64).eps * exp
+        assert api.average_density(torch.from_numpy(w).cuda()) == got   # same order every call
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gather_and_scatter_rows(gpu, dtype):
+    from dtfe_amd import _native as N, api
+    rng = np.random.default_rng(4)
+    src = torch.from_numpy(rng.standard_normal((5000, 3))).to("cuda", dtype)
+    idx = torch.from_numpy(rng.integers(0, 5000, 12000)).to("cuda", torch.int32)
+    assert torch.equal(api.gather_rows(src, idx), src[idx.long()])
+    v = src[:, 0].contiguous()
+    assert torch.equal(api.gather_rows(v, idx), v[idx.long()])
+    grads = torch.from_numpy(rng.standard_normal((12000, 3))).to("cuda", dtype)
+    got = api.scatter_add_rows(grads, idx, 5000)
+    exp = torch.zeros((5000, 3), dtype=torch.float64, device="cuda").index_add_(0, idx.long(), grads.double())
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert (got.double() - exp).abs().max().item() <= tol * exp.abs().max().item()
+    bad = idx.clone()
+    bad[7] = 5000
+    with pytest.raises(N.DTFEError):
+        api.gather_rows(src, bad)
+    with pytest.raises(N.DTFEError):
+        api.scatter_add_rows(grads, bad, 5000)
+
+
+def test_device_front_end_feeds_the_tessellation(gpu):
+    """prepare_particles on the device gives the host mirror's points, so the
+    same mesh; vertex data gathered on the device equals the host's."""
+    from dtfe_amd import api
+    ps = S.clustered(4000, 20.0, 20, 0.2, 1.0, 9, with_velocity=True, lognormal_weights=True)
+    host = M.prepare_particles(ps)
+    dev = api.prepare_particles(torch.from_numpy(ps.pos).cuda(), torch.from_numpy(ps.weight).cuda(), ps.box)
+    assert np.array_equal(dev.pos.cpu().numpy(), host.pos)
+    assert np.array_equal(dev.image_of.cpu().numpy(), host.image_of)
+    assert dev.padding == host.padding
+    assert abs(dev.rho_bar - host.rho_bar) <= 1e-14 * host.rho_bar
+    m = M.tessellate(host.pos)
+    vp = api.vertex_particles(dev, m)
+    w_dev = api.gather_rows(torch.from_numpy(ps.weight).cuda(), vp)
+    v_dev = api.gather_rows(torch.from_numpy(ps.vel).cuda(), vp)
+    assert np.array_equal(w_dev.cpu().numpy(), M.vertex_data(m, host.weight))
+    assert np.array_equal(v_dev.cpu().numpy(), M.vertex_data(m, host.vel))
+
+
+def test_periodic_images_at_scale(gpu):
+    """The clustered 128^3 particle set with its periodic padding: the
+    images equal the host restatement's, element for element."""
+    from dtfe_amd import api
+    ps = S.clustered_velocity()
+    L = ps.box[1] - ps.box[0]
+    pad = M.padding_length(ps.n, L)
+    padded = (-pad, L + pad, -pad, L + pad, -pad, L + pad)
+    ip, iof = api.periodic_images(torch.from_numpy(ps.pos).cuda(), ps.box, padded)
+    rp, rof = M.periodic_images(ps.pos, ps.box, padded)
+    print(f"images at scale: {ip.shape[0]} of {ps.n}")
+    assert np.array_equal(iof.cpu().numpy(), rof)
+    assert np.array_equal(ip.cpu().numpy(), rp)
