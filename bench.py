@@ -216,9 +216,11 @@ def main():
     if not args.no_adjoint:
         fa = fwd_adjoint(args, dm, w, vel, prep, box)
 
+    fe = front_end(ps, prep)
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, prep, m, box, n, mc_seed)
+        cpu = cpu_baseline(args, ps, prep, m, box, n, mc_seed)
 
     res = {
         "metric": "DTFE grid samples/s (256^3 grid, 8 MC samples per cell)",
@@ -252,6 +254,7 @@ def main():
                      "kernel": kernel, "algorithmic_bytes": B_alg,
                      "kernel_ms": k_sample_ms},
         "fp64_path": fp64,
+        "front_end_ms": fe,
         "fwd_adjoint_ratio": None if fa is None else fa["ratio"],
         "fwd_adjoint": fa,
         "cpu_baseline": cpu,
@@ -307,7 +310,36 @@ def fwd_adjoint(args, dm, w, vel, prep, box):
                         f"terms computed in torch)"}
 
 
-def cpu_baseline(args, prep, m, box, n, mc_seed):
+def front_end(ps, prep):
+    """DTFE()'s steps before the tessellation on the device: periodic images
+    of the particle set (the points the tessellation takes) and the average
+    density. Each call includes its host synchronisations (the image count
+    comes back to size the outputs)."""
+    import torch
+    from dtfe_amd import api, mesh as M
+    pos = torch.from_numpy(ps.pos).cuda()
+    wt = torch.from_numpy(ps.weight).cuda()
+    L = ps.box[1] - ps.box[0]
+    pad = M.padding_length(ps.n, L)
+    padded = tuple(ps.box[i] - pad if i % 2 == 0 else ps.box[i] + pad for i in range(6))
+
+    def timed(fn, k=5):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k, r
+
+    t_img, (ip, iof) = timed(lambda: api.periodic_images(pos, ps.box, padded))
+    t_rb, rb = timed(lambda: api.average_density(wt))
+    if ip.shape[0] != prep.pos.shape[0]:
+        raise RuntimeError("device periodic images disagree with the host's count")
+    return {"periodic_images": t_img, "average_density": t_rb, "particles": ps.n, "images": int(ip.shape[0])}
+
+
+def cpu_baseline(args, ps, prep, m, box, n, mc_seed):
     """The CPU oracle (the reference's algorithm restated in C, FP64): the
     whole grid split into x-slab partitions over worker threads, each with the
     reference's per-partition seed (seed + 1000 partNo, as its OpenMP run),
@@ -320,6 +352,11 @@ def cpu_baseline(args, prep, m, box, n, mc_seed):
     t0 = time.perf_counter()
     rho, _, _ = O.vertex_density(m, w, prep.rho_bar)
     t_vd = time.perf_counter() - t0
+    # the periodic images of the particle set (findParticlesInBox), one thread
+    padded = tuple(ps.box[i] - prep.padding if i % 2 == 0 else ps.box[i] + prep.padding for i in range(6))
+    t0 = time.perf_counter()
+    O.periodic_images(ps.pos, ps.box, padded)
+    t_img = time.perf_counter() - t0
     fields = ("density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity")
     dx = (box[1] - box[0]) / n[0]
 
@@ -350,6 +387,7 @@ def cpu_baseline(args, prep, m, box, n, mc_seed):
             "single_thread": {"value": ns1 / t1, "samples": ns1, "seconds": t1,
                               "sample": f"{slabs1} x-slab(s), one partition"},
             "vertex_density_seconds_one_thread": t_vd,
+            "periodic_images_seconds_one_thread": t_img,
             "host_cpu": platform.processor() or platform.machine(),
             "host_threads_available": os.cpu_count()}
 

@@ -156,33 +156,6 @@ Real averageDensity(std::vector<Particle_data> &p, const User_options &o) {
     return Real(total / o.boxCoordinates.volume());
 }
 
-// findParticlesInBox / insertParticlesInBox (src/subpartition.h:79-154)
-void findParticlesInBox(std::vector<Particle_data> &p, std::vector<Particle_data> *out, const User_options &o) {
-    const Box &box = o.paddedBox;
-    const Box &full = o.boxCoordinates;
-    Real L[NO_DIM];
-    for (size_t i = 0; i < NO_DIM; ++i) L[i] = full[2 * i + 1] - full[2 * i];
-    out->clear();
-    if (!o.periodic) {
-        for (auto &q : p)
-            if (box.isParticleInBox(q)) out->push_back(q);
-        return;
-    }
-    for (int i1 = 0; i1 < 3; ++i1)
-        for (int i2 = 0; i2 < 3; ++i2)
-            for (int i3 = 0; i3 < 3; ++i3) {
-                Real off[3] = {Real(i1 - 1) * L[0], Real(i2 - 1) * L[1], Real(i3 - 1) * L[2]};
-                Box nb = box;
-                nb.translate(off);
-                for (auto &q : p)
-                    if (nb.isParticleInBox(q)) {
-                        Particle_data t = q;
-                        for (int j = 0; j < NO_DIM; ++j) t.pos[j] -= off[j];
-                        out->push_back(t);
-                    }
-            }
-}
-
 // ---------------------------------------------------------------------------
 // device memory
 // ---------------------------------------------------------------------------
@@ -240,6 +213,46 @@ struct Ctx {
 };
 
 constexpr int kDtype = sizeof(Real) == 8 ? DTFE_F64 : DTFE_F32;
+
+// findParticlesInBox / insertParticlesInBox (src/subpartition.h:79-154) with
+// the box tests and the shifts on the device (dtfe_hip_periodic_images): for
+// each of the 27 translates in the reference's order, the particles in input
+// order that fall in the translated padded box, shifted back; non-periodic:
+// the particles inside the padded box. The particle records are copied on the
+// host, from the particle index of each image.
+void findParticlesInBoxDevice(std::vector<Particle_data> &p, std::vector<Particle_data> *out, const User_options &o) {
+    const size_t n = p.size();
+    out->clear();
+    if (n == 0) return;
+    std::vector<Real> hpos(3 * n);
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) hpos[3 * i + k] = p[i].pos[k];
+    Ctx ctx(o.device);
+    DevMem d_pos = upload(hpos.data(), hpos.size());
+    double full[6], padded[6];
+    for (int i = 0; i < 6; ++i) {
+        full[i] = (double)o.boxCoordinates[i];
+        padded[i] = (double)o.paddedBox[i];
+    }
+    int64_t count = 0;
+    ctx.check(dtfe_hip_periodic_images(ctx.h, (int64_t)n, kDtype, d_pos.p, full, padded, o.periodic ? 1 : 0, 0,
+                                       nullptr, nullptr, &count),
+              "periodic images");
+    DevMem d_img(3 * (size_t)count * sizeof(Real)), d_of((size_t)count * sizeof(int32_t));
+    ctx.check(dtfe_hip_periodic_images(ctx.h, (int64_t)n, kDtype, d_pos.p, full, padded, o.periodic ? 1 : 0, count,
+                                       d_img.p, d_of.as<int32_t>(), &count),
+              "periodic images");
+    std::vector<Real> img;
+    std::vector<int32_t> of;
+    download(img, d_img, 3 * (size_t)count);
+    download(of, d_of, (size_t)count);
+    out->reserve((size_t)count);
+    for (int64_t k = 0; k < count; ++k) {
+        Particle_data t = p[(size_t)of[k]];
+        for (int j = 0; j < NO_DIM; ++j) t.pos[j] = img[3 * k + j];
+        out->push_back(t);
+    }
+}
 
 unsigned field_bits(const Field &f) {
     unsigned b = 0;
@@ -489,7 +502,7 @@ void DTFE(std::vector<Particle_data> *allParticles, std::vector<Sample_point> &s
         }
         t.paddedBox = t.region;
         t.paddedBox.addPadding(t.paddingLength);
-        findParticlesInBox(*pp, &regionParts, t);
+        findParticlesInBoxDevice(*pp, &regionParts, t);
         t.periodic = false;
         t.updateFullBox(t.region);
         pp->clear();
@@ -501,7 +514,7 @@ void DTFE(std::vector<Particle_data> *allParticles, std::vector<Sample_point> &s
         t.paddedBox = t.region;
         t.paddedBox.addPadding(t.paddingLength);
         if (userOptions.periodic) {
-            findParticlesInBox(*pp, &padded, t);
+            findParticlesInBoxDevice(*pp, &padded, t);
             pp->clear();
             pp = &padded;
         }

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -62,6 +63,8 @@ constexpr int kMaxScalars = 4;
 // keep one row's MT words (3 per sample) in a 64 KB LDS ring
 constexpr int kMaxSamples = 4096;
 constexpr int kMaxWalkSteps = 1 << 20;
+// workgroups of the exact-sign pass when its row count stays on the device
+constexpr int kExactGrid = 256;
 // Workgroup size of the one-thread-per-sample Monte-Carlo kernel.
 #ifndef DTFE_LANES_BLOCK
 #define DTFE_LANES_BLOCK 64
@@ -1415,12 +1418,14 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_mc(SampleArgs a) {
     }
 }
 
-// Deferred rows, redone with exact signs (symbolic tie-break).
+// Deferred rows, redone with exact signs (symbolic tie-break). n_deferred < 0:
+// the count is the queue's, read on the device (no host round trip).
 template <typename F, bool VEL, int NS, bool ADJ>
 __global__ void DTFE_SAMPLE_BOUNDS k_sample_exact(SampleArgs a, int n_deferred) {
     int stride = gridDim.x * blockDim.x;
     const int64_t wpr = 3 * (int64_t)a.M;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_deferred; i += stride) {
+    const int n = n_deferred >= 0 ? n_deferred : min(a.status[1], a.defer_cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         int64_t row = a.defer_rows[i];
         SavedWords words{a.defer_words + (int64_t)i * wpr, wpr * row};
         run_row<F, VEL, NS, true, ADJ>(a, row, words);
@@ -2276,7 +2281,8 @@ int setup_tables(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_mc *mc, SampleArg
 }
 
 // MC plan, main launch, deferred rows, status check.
-int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool vel, int ns, bool adj) {
+int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool vel, int ns, bool adj,
+                bool exact_all = false) {
     int rc;
     const int64_t n_rows = a.n_rows;
     int grid;
@@ -2331,6 +2337,12 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     // for the exact kernel, with their MC words
     const int64_t wpr = is_mc ? 3 * (int64_t)a.M : 0;
     int64_t cap = std::min<int64_t>(n_rows, std::max<int64_t>(1 << 16, n_rows / 16));
+    // DTFE_HIP_DEFER_CAP (testing): a smaller queue, so that the overflow
+    // path (every row redone with exact signs) can be exercised
+    if (const char *e = std::getenv("DTFE_HIP_DEFER_CAP")) {
+        const long v = std::atol(e);
+        if (v > 0) cap = std::min<int64_t>(cap, v);
+    }
     if ((rc = ensure(ctx, ctx->defer_rows, cap * sizeof(int64_t)))) return rc;
     if (wpr && (rc = ensure(ctx, ctx->defer_words, cap * wpr * sizeof(uint32_t)))) return rc;
     a.defer_rows = (int64_t *)ctx->defer_rows.p;
@@ -2349,10 +2361,19 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
             else launch_lanes<float, false>(a, (const int32_t *)ctx->centre.p, ns, a.M, grid, lds_lanes, ctx->stream);
         }
     } else {
-        launch_any(a, f64, vel, ns, adj, 0, is_mc, grid, lds, 0, ctx->stream);
+        launch_any(a, f64, vel, ns, adj, exact_all ? 2 : 0, is_mc, grid, lds, 0, ctx->stream);
     }
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
     HIP_TRY(hipGetLastError());
+    if (adj) {
+        // the adjoint does not wait: deferred rows follow on the device with
+        // the queue's count, and the caller reads the status after the vertex
+        // chain (a queue overflow then redoes the whole pass with exact_all)
+        if (!exact_all) launch_any(a, f64, vel, ns, adj, 1, is_mc, kExactGrid, 0, -1, ctx->stream);
+        HIP_TRY(hipGetLastError());
+        ctx->last_ms[0] = ctx->last_ms[1] = ctx->last_ms[4] = 0.0;
+        return DTFE_OK;
+    }
     int st[8];
     HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2382,19 +2403,7 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     ctx->last_ms[1] = 0.0;
     ctx->last_deferred = st[1];
     if (st[2]) {
-        // queue overflow: redo everything with exact signs; the adjoint first
-        // drops what the certified rows already scattered
-        if (adj) {
-            const int64_t nv = a.nv;
-            auto clear = [&](void *p, int64_t n) {
-                if (!p) return;
-                if (f64) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (double *)p, n, 0.0);
-                else hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, (float *)p, n, 0.0f);
-            };
-            clear(a.d_rho, nv);
-            clear(a.d_vel, 3 * nv);
-            clear(a.d_pos, 3 * nv);
-        }
+        // queue overflow: redo everything with exact signs
         launch_any(a, f64, vel, ns, adj, 2, is_mc, grid, lds, 0, ctx->stream);
     } else if (st[1] > 0) {
         HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -2742,8 +2751,8 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: gradients are zero");
     }
     if ((rc = ensure(ctx, ctx->adj_rho, nv * esz))) return rc;
-    {
-        // every accumulator starts at zero (one launch)
+    // every accumulator starts at zero (one launch)
+    auto zero_all = [&]() -> int {
         const int64_t n0 = gr->position ? 3 * nv : 0, n1 = gr->weight ? nv : 0, n2 = gr->velocity ? 3 * nv : 0;
         const int g = grid_for(n0 + n1 + n2 + nv);
         if (f64)
@@ -2752,7 +2761,10 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         else
             hipLaunchKernelGGL(k_zero4<float>, dim3(g), dim3(kBlock), 0, ctx->stream, (float *)gr->position, n0,
                                (float *)gr->weight, n1, (float *)gr->velocity, n2, (float *)ctx->adj_rho.p, nv);
-    }
+        HIP_TRY(hipGetLastError());
+        return DTFE_OK;
+    };
+    if ((rc = zero_all())) return rc;
 
     SampleArgs a{};
     a.pos = m->pos;
@@ -2784,11 +2796,26 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true))) return rc;
     }
     // vertex density chain
-    if (up->density && (gr->weight || gr->position)) {
+    const bool chain = up->density && (gr->weight || gr->position);
+    const int parts = std::min(1024, grid_for(nv));
+    int *need_W = nullptr;
+    if (chain) {
         if ((rc = ensure(ctx, ctx->adj_dW, nv * sizeof(double)))) return rc;
-        const int parts = std::min(1024, grid_for(nv));
         if ((rc = ensure(ctx, ctx->adj_partial, parts * sizeof(double) + 64))) return rc;
-        int *need_W = (int *)((char *)ctx->adj_partial.p + parts * sizeof(double));
+        need_W = (int *)((char *)ctx->adj_partial.p + parts * sizeof(double));
+    }
+    auto adj_tets = [&](const double *dW) {
+        if (!gr->position) return;
+        const dim3 g(grid_for((nc + kAdjChunk - 1) / kAdjChunk));
+        if (f64)
+            hipLaunchKernelGGL(k_adj_tets<double>, g, dim3(kBlock), 0, ctx->stream, m->pos, (const int4 *)m->cells,
+                               nc, dW, (double *)gr->position);
+        else
+            hipLaunchKernelGGL(k_adj_tets<float>, g, dim3(kBlock), 0, ctx->stream, m->pos, (const int4 *)m->cells,
+                               nc, dW, (float *)gr->position);
+    };
+    auto launch_chain = [&]() {
+        if (!chain) return;
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ctx->stream, need_W, (int64_t)1, 0);
         if (f64)
             hipLaunchKernelGGL(k_adj_vertex_rho<double>, dim3(parts), dim3(kBlock), 0, ctx->stream,
@@ -2800,25 +2827,35 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
                                (const float *)weight, (const float *)vf->density, (const float *)ctx->adj_rho.p,
                                nv, rho_bar, (float *)gr->weight, (double *)ctx->adj_dW.p,
                                (double *)ctx->adj_partial.p, need_W);
-        auto adj_tets = [&](const double *dW) {
-            if (!gr->position) return;
-            const dim3 g(grid_for((nc + kAdjChunk - 1) / kAdjChunk));
-            if (f64)
-                hipLaunchKernelGGL(k_adj_tets<double>, g, dim3(kBlock), 0, ctx->stream, m->pos, (const int4 *)m->cells,
-                                   nc, dW, (double *)gr->position);
-            else
-                hipLaunchKernelGGL(k_adj_tets<float>, g, dim3(kBlock), 0, ctx->stream, m->pos, (const int4 *)m->cells,
-                                   nc, dW, (float *)gr->position);
-        };
         adj_tets((const double *)ctx->adj_dW.p);
-        std::vector<double> part(parts + 1);
-        auto read_partials = [&]() -> int {
+    };
+    // the one wait of the call: the rows' status, then the chain's partial sums
+    int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<double> part(parts + 1);
+    auto read_back = [&]() -> int {
+        if (a.n_rows > 0)
+            HIP_TRY(hipMemcpyAsync(st, ctx->status.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+        if (chain)
             HIP_TRY(hipMemcpyAsync(part.data(), ctx->adj_partial.p, parts * sizeof(double) + sizeof(int),
                                    hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-            return DTFE_OK;
-        };
-        if ((rc = read_partials())) return rc;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return DTFE_OK;
+    };
+    launch_chain();
+    if ((rc = read_back())) return rc;
+    if (st[2]) {
+        // deferred-row queue overflow: redo the rows with exact signs from
+        // zeroed accumulators, then the chain
+        if ((rc = zero_all())) return rc;
+        if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true, true))) return rc;
+        launch_chain();
+        if ((rc = read_back())) return rc;
+    }
+    ctx->last_deferred = st[1];
+    if (st[0] > 0)
+        return fail(ctx, DTFE_ERR_WALK,
+                    std::to_string(st[0]) + " point locations did not terminate: the mesh is not a valid triangulation");
+    if (chain) {
         int need = 0;
         std::memcpy(&need, part.data() + parts, sizeof(int));
         if (need) {
@@ -2849,7 +2886,9 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
             hipLaunchKernelGGL(k_diff, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream, (double *)ctx->adj_dW.p,
                                (const double *)ctx->adj_dW2.p, (const double *)ctx->adj_dW.p, nv);
             adj_tets((const double *)ctx->adj_dW.p);
-            if ((rc = read_partials())) return rc;
+            HIP_TRY(hipMemcpyAsync(part.data(), ctx->adj_partial.p, parts * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
         }
         double sum = 0.0;
         for (int i = 0; i < parts; ++i) sum += part[i];

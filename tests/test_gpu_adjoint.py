@@ -246,3 +246,46 @@ def test_adjoint_monte_carlo_matches_fp64_autograd(gpu, kind):
         assert err < 1e-9, name
     err_rb = abs(g_rb - rb.grad.item()) / abs(rb.grad.item())
     assert err_rb < 1e-9
+
+
+@pytest.mark.parametrize("cap", [None, "1"])
+def test_adjoint_tied_rows(gpu, cap, monkeypatch):
+    """Cell centres exactly on mesh vertices: their rows need exact signs.
+    The adjoint queues them and redoes them on the device (with a one-row
+    queue: the overflow path, every row redone exactly from zeroed
+    accumulators), and still matches FP64 autograd."""
+    from dtfe_amd import api
+    if cap:
+        monkeypatch.setenv("DTFE_HIP_DEFER_CAP", cap)
+    prob = P.snapped_problem()
+    m = prob.mesh
+    n = (12, 12, 12)
+    spec = api.GridSpec(prob.region, n)
+    dm, w, v, rho = _setup(prob, torch.float64)
+    fields = ["density", "velocity", "velocity_divergence"]
+    out = api.sample(dm, rho, v, None, fields, spec, method=0, want_cells=True)
+    located = out["located_cell"].cpu().numpy().astype(np.int64)
+    assert (located >= 0).all()
+    rng = np.random.default_rng(17)
+    up = {f: torch.from_numpy(rng.standard_normal(tuple(out[f].shape))).cuda() for f in fields}
+    g, g_rb = api.adjoint(dm, rho, w, v, prob.rho_bar, spec, up, method=0)
+    deferred = api.context(dm.device).last_deferred()
+    print(f"adjoint tied rows: {deferred} deferred (queue cap {cap or 'default'})")
+    assert deferred > 1
+
+    pos = torch.from_numpy(m.pos).clone().requires_grad_(True)
+    wt = torch.from_numpy(prob.w).clone().requires_grad_(True)
+    vt = torch.from_numpy(prob.vel).clone().requires_grad_(True)
+    _, _, hull = O.vertex_density(m, prob.w, prob.rho_bar)
+    rb = torch.tensor(prob.rho_bar, dtype=torch.float64, requires_grad=True)
+    d, vel, G = R.forward(pos, wt, vt, torch.from_numpy(m.cells).long(), torch.from_numpy(hull),
+                          torch.from_numpy(located), torch.from_numpy(_grid(prob, n)), rb, True)
+    div = G[:, 0] + G[:, 4] + G[:, 8]
+    L = ((d * up["density"].cpu()).sum() + (vel * up["velocity"].cpu()).sum() +
+         (div * up["velocity_divergence"].cpu()).sum())
+    L.backward()
+    for name, ref in (("position", pos.grad), ("weight", wt.grad), ("velocity", vt.grad)):
+        err = (g[name].cpu() - ref).abs().max().item() / ref.abs().max().item()
+        print(f"adjoint tied rows {name}: max rel err {err:.3e}")
+        assert err < 1e-9, name
+    assert abs(g_rb - rb.grad.item()) <= 1e-9 * abs(rb.grad.item())

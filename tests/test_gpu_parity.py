@@ -341,3 +341,28 @@ def test_samples_outside_hull(gpu, samples):
     assert np.all((np.round(k) >= 1) & (np.round(k) <= samples))
     assert np.any(np.round(k) < samples), "no cell straddles the hull"
     assert np.any(np.round(k) == samples)
+
+
+@pytest.mark.parametrize("method,samples", [(0, 1), (2, 3), (2, 4)])
+def test_deferral_queue_overflow(gpu, method, samples, monkeypatch):
+    """Cell centres on mesh vertices need exact signs. With a one-row queue
+    the deferred rows overflow it and every row is redone with exact signs;
+    the results are the oracle's either way."""
+    from dtfe_amd import api
+    prob = P.snapped_problem()
+    dm, rho, vel, _ = gpu_setup(prob, torch.float64)
+    n = (12, 12, 12)
+    fields = ["density", "velocity_gradient"]
+    ref = O.sample(prob.mesh, oracle_rho(prob), prob.vel, None, fields=tuple(fields), kind="grid",
+                   box=prob.region, n=n, method=method, samples=samples, seed=4)
+    for cap in (None, "1"):
+        if cap:
+            monkeypatch.setenv("DTFE_HIP_DEFER_CAP", cap)
+        out = api.sample(dm, rho, vel, None, fields, api.GridSpec(prob.region, n), method=method,
+                         samples=samples, seed=4, want_cells=True)
+        deferred = api.context(dm.device).last_deferred()
+        print(f"method={method} M={samples} cap={cap}: {deferred} deferred")
+        assert deferred > 1
+        assert np.array_equal(out["located_cell"].cpu().numpy(), ref["located_cell"])
+        for f in fields:
+            assert rel_err(out[f].cpu().numpy(), ref[f]) < TOL64, (cap, f)
