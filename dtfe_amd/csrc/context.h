@@ -27,7 +27,7 @@ struct dtfe_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, plan, flags, rows, mass, mom,
-        defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, centre, records,
+        defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, adj_acc, centre, records,
         front;  // scratch of the front-end calls (counts, offsets, partial sums, flags)
     int last_deferred = 0;
     int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples

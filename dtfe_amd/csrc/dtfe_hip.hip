@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -585,10 +586,14 @@ struct SampleArgs {
     // adjoint: upstream gradients per row and per-vertex accumulators
     const void *g_density, *g_velocity, *g_grad, *g_div, *g_shear, *g_vort;
     const int32_t *start_cell;  // optional walk start per row (the forward's located cells)
-    void *d_rho;   // [nv] dL/d rho_v
-    void *d_vel;   // [nv][3]
-    void *d_pos;   // [nv][3]
+    // per-vertex sums, one 8-value record per vertex: dL/d rho_v, dL/dv_v
+    // (3), dL/dx_v (3), unused; the seven atomics of a vertex land in one
+    // 32-byte (FP32) or 64-byte (FP64) record
+    void *acc;     // [nv][8]
+    bool want_vel, want_pos;
 };
+
+constexpr int kAccStride = 8;
 
 __device__ __forceinline__ int seed_cell(const SeedGrid &sg, d3 p) {
     const double *b = sg.bbox;
@@ -840,27 +845,32 @@ struct VertexGrad {
 struct GlobalSink {
     template <typename F>
     __device__ __forceinline__ void add(const SampleArgs &a, int v, const VertexGrad &d) const {
-        if (d.r != 0.0) atomic_add_f<F>(a.d_rho, v, (F)d.r);
-        if (a.d_vel)
+        const int64_t o = (int64_t)kAccStride * v;
+        if (d.r != 0.0) atomic_add_f<F>(a.acc, o, (F)d.r);
+        if (a.want_vel)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                if (d.v[c] != 0.0) atomic_add_f<F>(a.d_vel, 3 * (int64_t)v + c, (F)d.v[c]);
-        if (a.d_pos)
+                if (d.v[c] != 0.0) atomic_add_f<F>(a.acc, o + 1 + c, (F)d.v[c]);
+        if (a.want_pos)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                if (d.x[c] != 0.0) atomic_add_f<F>(a.d_pos, 3 * (int64_t)v + c, (F)d.x[c]);
+                if (d.x[c] != 0.0) atomic_add_f<F>(a.acc, o + 4 + c, (F)d.x[c]);
     }
 };
 
 // merged per vertex in an LDS hash table of the workgroup (FP64), flushed to
 // HBM once per workgroup pass; a vertex that finds no slot within a few
-// probes goes straight to HBM
+// probes goes straight to HBM. The flush sorts the slots by vertex id first:
+// Morton-numbered vertices of one brick then come in runs of nearby ids, and
+// a wave's atomics (8 vertices x 8 values) touch a few consecutive records
+// instead of 64 scattered cache lines.
 constexpr int kSinkSlots = 512;
 constexpr int kSinkVals = 7;
 
 struct LdsSink {
     int *keys;      // [kSinkSlots], -1 = free
     double *vals;   // [kSinkSlots][kSinkVals]
+    int *order;     // [kSinkSlots] flush order
     __device__ void clear() const {
         for (int i = threadIdx.x; i < kSinkSlots; i += blockDim.x) {
             keys[i] = -1;
@@ -887,13 +897,39 @@ struct LdsSink {
         }
         GlobalSink().add<F>(a, v, d);
     }
+    // Every thread of the workgroup calls it after a barrier; bitonic sort of
+    // (key, slot) with free slots last, then one atomic per nonzero value,
+    // lane = (sorted slot, value).
     template <typename F>
     __device__ void flush(const SampleArgs &a) const {
         for (int i = threadIdx.x; i < kSinkSlots; i += blockDim.x) {
-            const int v = keys[i];
-            if (v < 0) continue;
-            const double *o = vals + i * kSinkVals;
-            GlobalSink().add<F>(a, v, VertexGrad{o[0], {o[1], o[2], o[3]}, {o[4], o[5], o[6]}});
+            order[i] = i;
+            if (keys[i] < 0) keys[i] = INT_MAX;
+        }
+        __syncthreads();
+        for (int k = 2; k <= kSinkSlots; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = threadIdx.x; t < kSinkSlots / 2; t += blockDim.x) {
+                    const int i = 2 * t - (t & (j - 1)), l = i + j;
+                    const bool up = (i & k) == 0;
+                    const int ki = keys[i], kl = keys[l];
+                    if ((ki > kl) == up) {
+                        keys[i] = kl;
+                        keys[l] = ki;
+                        const int oi = order[i];
+                        order[i] = order[l];
+                        order[l] = oi;
+                    }
+                }
+                __syncthreads();
+            }
+        for (int e = threadIdx.x; e < kSinkSlots * kAccStride; e += blockDim.x) {
+            const int sl = e / kAccStride, c = e % kAccStride;
+            const int v = keys[sl];
+            if (v == INT_MAX || c >= kSinkVals) continue;
+            if ((c >= 1 && c <= 3 && !a.want_vel) || (c >= 4 && !a.want_pos)) continue;
+            const double x = vals[order[sl] * kSinkVals + c];
+            if (x != 0.0) atomic_add_f<F>(a.acc, (int64_t)kAccStride * v + c, (F)x);
         }
     }
 };
@@ -1362,7 +1398,8 @@ template <typename F, bool VEL, int KIND>
 __global__ void __launch_bounds__(kBlock, DTFE_ADJ_MIN_WAVES) k_adjoint_rows(SampleArgs a, BrickGrid bg) {
     __shared__ int keys[kSinkSlots];
     __shared__ double vals[kSinkSlots * kSinkVals];
-    const LdsSink sink{keys, vals};
+    __shared__ int order[kSinkSlots];
+    const LdsSink sink{keys, vals, order};
     NoWords nw;
     if (KIND != DTFE_SAMPLE_POINTS) {
         sink.clear();
@@ -1684,6 +1721,23 @@ __global__ void k_adj_vertex(const F *__restrict__ w, const F *__restrict__ rho,
 // d rho_v / d W_v = -rho_v / W_v. A vertex with rho_v = 0 is a hull vertex
 // (rho fixed at 0: no gradient) or has w_v = 0, whose d rho_v / d w_v needs
 // W_v: such a vertex raises *need_W and the caller takes the volume path.
+// The adjoint rows' per-vertex sums out of their packed records: dL/d rho
+// into the vertex chain's buffer, dL/dv and dL/dx into the caller's arrays
+// (NULL: not wanted).
+template <typename F>
+__global__ void __launch_bounds__(kBlock) k_unpack_grads(const F *__restrict__ acc, int64_t nv, F *__restrict__ d_rho,
+                                                         F *__restrict__ d_vel, F *__restrict__ d_pos) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
+        const F *r = acc + (int64_t)kAccStride * v;
+        d_rho[v] = r[0];
+        if (d_vel)
+            for (int c = 0; c < 3; ++c) d_vel[3 * v + c] = r[1 + c];
+        if (d_pos)
+            for (int c = 0; c < 3; ++c) d_pos[3 * v + c] = r[4 + c];
+    }
+}
+
 template <typename F>
 __global__ void k_adj_vertex_rho(const F *__restrict__ w, const F *__restrict__ rho, const F *__restrict__ d_rho,
                                  int64_t nv, double rho_bar, F *__restrict__ d_w, double *__restrict__ dW,
@@ -2458,7 +2512,7 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_b, &ctx->bbox_part,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
-                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->centre, &ctx->records, &ctx->front};
+                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->front};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 9; ++i)
@@ -2751,18 +2805,30 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: gradients are zero");
     }
     if ((rc = ensure(ctx, ctx->adj_rho, nv * esz))) return rc;
-    // every accumulator starts at zero (one launch)
+    if ((rc = ensure(ctx, ctx->adj_acc, (size_t)kAccStride * nv * esz))) return rc;
+    // the packed per-vertex sums and the weight gradient start at zero (one
+    // launch); the other outputs are written whole by k_unpack_grads
     auto zero_all = [&]() -> int {
-        const int64_t n0 = gr->position ? 3 * nv : 0, n1 = gr->weight ? nv : 0, n2 = gr->velocity ? 3 * nv : 0;
-        const int g = grid_for(n0 + n1 + n2 + nv);
+        const int64_t n0 = (int64_t)kAccStride * nv, n1 = gr->weight ? nv : 0;
+        const int g = grid_for(n0 + n1);
         if (f64)
-            hipLaunchKernelGGL(k_zero4<double>, dim3(g), dim3(kBlock), 0, ctx->stream, (double *)gr->position, n0,
-                               (double *)gr->weight, n1, (double *)gr->velocity, n2, (double *)ctx->adj_rho.p, nv);
+            hipLaunchKernelGGL(k_zero4<double>, dim3(g), dim3(kBlock), 0, ctx->stream, (double *)ctx->adj_acc.p, n0,
+                               (double *)gr->weight, n1, (double *)nullptr, (int64_t)0, (double *)nullptr, (int64_t)0);
         else
-            hipLaunchKernelGGL(k_zero4<float>, dim3(g), dim3(kBlock), 0, ctx->stream, (float *)gr->position, n0,
-                               (float *)gr->weight, n1, (float *)gr->velocity, n2, (float *)ctx->adj_rho.p, nv);
+            hipLaunchKernelGGL(k_zero4<float>, dim3(g), dim3(kBlock), 0, ctx->stream, (float *)ctx->adj_acc.p, n0,
+                               (float *)gr->weight, n1, (float *)nullptr, (int64_t)0, (float *)nullptr, (int64_t)0);
         HIP_TRY(hipGetLastError());
         return DTFE_OK;
+    };
+    auto unpack = [&]() {
+        if (f64)
+            hipLaunchKernelGGL(k_unpack_grads<double>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                               (const double *)ctx->adj_acc.p, nv, (double *)ctx->adj_rho.p, (double *)gr->velocity,
+                               (double *)gr->position);
+        else
+            hipLaunchKernelGGL(k_unpack_grads<float>, dim3(grid_for(nv)), dim3(kBlock), 0, ctx->stream,
+                               (const float *)ctx->adj_acc.p, nv, (float *)ctx->adj_rho.p, (float *)gr->velocity,
+                               (float *)gr->position);
     };
     if ((rc = zero_all())) return rc;
 
@@ -2785,9 +2851,9 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
     a.g_shear = up->velocity_shear;
     a.g_vort = up->velocity_vorticity;
     a.start_cell = up->start_cell;
-    a.d_rho = ctx->adj_rho.p;
-    a.d_vel = gr->velocity;
-    a.d_pos = gr->position;
+    a.acc = ctx->adj_acc.p;
+    a.want_vel = gr->velocity != nullptr;
+    a.want_pos = gr->position != nullptr;
     std::vector<double> axes, trig;
     if ((rc = setup_rows(ctx, smp, method, a, axes, trig))) return rc;
     if (a.n_rows > 0) {
@@ -2795,6 +2861,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
         if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true))) return rc;
     }
+    unpack();
     // vertex density chain
     const bool chain = up->density && (gr->weight || gr->position);
     const int parts = std::min(1024, grid_for(nv));
@@ -2848,6 +2915,7 @@ int dtfe_hip_adjoint(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields
         // zeroed accumulators, then the chain
         if ((rc = zero_all())) return rc;
         if ((rc = launch_rows(ctx, a, mc, f64, vel_up, 0, true, true))) return rc;
+        unpack();
         launch_chain();
         if ((rc = read_back())) return rc;
     }
