@@ -860,17 +860,15 @@ struct GlobalSink {
 
 // merged per vertex in an LDS hash table of the workgroup (FP64), flushed to
 // HBM once per workgroup pass; a vertex that finds no slot within a few
-// probes goes straight to HBM. The flush sorts the slots by vertex id first:
-// Morton-numbered vertices of one brick then come in runs of nearby ids, and
-// a wave's atomics (8 vertices x 8 values) touch a few consecutive records
-// instead of 64 scattered cache lines.
+// probes goes straight to HBM. The flush gives each lane one value of one
+// slot, so a wave's atomics land in 8 records (8 values each) rather than in
+// 64 scattered cache lines.
 constexpr int kSinkSlots = 512;
 constexpr int kSinkVals = 7;
 
 struct LdsSink {
     int *keys;      // [kSinkSlots], -1 = free
     double *vals;   // [kSinkSlots][kSinkVals]
-    int *order;     // [kSinkSlots] flush order
     __device__ void clear() const {
         for (int i = threadIdx.x; i < kSinkSlots; i += blockDim.x) {
             keys[i] = -1;
@@ -897,38 +895,16 @@ struct LdsSink {
         }
         GlobalSink().add<F>(a, v, d);
     }
-    // Every thread of the workgroup calls it after a barrier; bitonic sort of
-    // (key, slot) with free slots last, then one atomic per nonzero value,
-    // lane = (sorted slot, value).
+    // Every thread of the workgroup calls it after a barrier: one atomic per
+    // nonzero value, lane = (slot, value).
     template <typename F>
     __device__ void flush(const SampleArgs &a) const {
-        for (int i = threadIdx.x; i < kSinkSlots; i += blockDim.x) {
-            order[i] = i;
-            if (keys[i] < 0) keys[i] = INT_MAX;
-        }
-        __syncthreads();
-        for (int k = 2; k <= kSinkSlots; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int t = threadIdx.x; t < kSinkSlots / 2; t += blockDim.x) {
-                    const int i = 2 * t - (t & (j - 1)), l = i + j;
-                    const bool up = (i & k) == 0;
-                    const int ki = keys[i], kl = keys[l];
-                    if ((ki > kl) == up) {
-                        keys[i] = kl;
-                        keys[l] = ki;
-                        const int oi = order[i];
-                        order[i] = order[l];
-                        order[l] = oi;
-                    }
-                }
-                __syncthreads();
-            }
         for (int e = threadIdx.x; e < kSinkSlots * kAccStride; e += blockDim.x) {
             const int sl = e / kAccStride, c = e % kAccStride;
             const int v = keys[sl];
-            if (v == INT_MAX || c >= kSinkVals) continue;
+            if (v < 0 || c >= kSinkVals) continue;
             if ((c >= 1 && c <= 3 && !a.want_vel) || (c >= 4 && !a.want_pos)) continue;
-            const double x = vals[order[sl] * kSinkVals + c];
+            const double x = vals[sl * kSinkVals + c];
             if (x != 0.0) atomic_add_f<F>(a.acc, (int64_t)kAccStride * v + c, (F)x);
         }
     }
@@ -1398,8 +1374,7 @@ template <typename F, bool VEL, int KIND>
 __global__ void __launch_bounds__(kBlock, DTFE_ADJ_MIN_WAVES) k_adjoint_rows(SampleArgs a, BrickGrid bg) {
     __shared__ int keys[kSinkSlots];
     __shared__ double vals[kSinkSlots * kSinkVals];
-    __shared__ int order[kSinkSlots];
-    const LdsSink sink{keys, vals, order};
+    const LdsSink sink{keys, vals};
     NoWords nw;
     if (KIND != DTFE_SAMPLE_POINTS) {
         sink.clear();
@@ -1505,15 +1480,18 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_exact_all(SampleArgs a) {
 // ----------------------------------------------------------------------------
 // vertex density
 // ----------------------------------------------------------------------------
-// One thread per tetrahedron. The 4 (vertex, volume) pairs of a workgroup's
-// 256 tetrahedra are first merged in an LDS hash table (Morton-ordered
-// meshes make them share ~4x fewer distinct vertices), then each distinct
-// vertex gets one FP64 atomic in HBM.
+// kVolChunk tetrahedra per thread between flushes. The 4 (vertex, volume)
+// pairs of a workgroup's tetrahedra are first merged in an LDS hash table
+// (Morton-ordered meshes make them share far fewer distinct vertices), then
+// each distinct vertex gets one FP64 atomic in HBM.
 #ifndef DTFE_VOL_CHUNK
-#define DTFE_VOL_CHUNK 1
+#define DTFE_VOL_CHUNK 4
 #endif
-constexpr int kVolChunk = DTFE_VOL_CHUNK;     // tetrahedra per thread between flushes
-constexpr int kAggSlots = 2048 * kVolChunk;  // load factor <= 0.5
+#ifndef DTFE_VOL_SLOTS
+#define DTFE_VOL_SLOTS 1024
+#endif
+constexpr int kVolChunk = DTFE_VOL_CHUNK;  // tetrahedra per thread between flushes
+constexpr int kAggSlots = DTFE_VOL_SLOTS;  // a vertex that finds no slot goes straight to HBM
 
 __global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restrict__ pos,
                                                          const int4 *__restrict__ cells,
@@ -1541,14 +1519,16 @@ __global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restric
             for (int q = 0; q < 4; ++q) {
                 const int v = id[q];
                 unsigned slot = (unsigned)(((uint64_t)((unsigned)v * 2654435761u) * kAggSlots) >> 32);
-                for (int probe = 0; probe < kAggSlots; ++probe) {
+                bool done = false;
+                for (int probe = 0; probe < 32 && !done; ++probe) {
                     int k = atomicCAS(&keys[slot], -1, v);
                     if (k == -1 || k == v) {
                         atomicAdd(&vals[slot], vol);
-                        break;
+                        done = true;
                     }
                     slot = slot + 1 == (unsigned)kAggSlots ? 0u : slot + 1;
                 }
+                if (!done) atomicAdd(W + v, vol);
             }
             int4 nb = nbrs[t];
             // a face on the hull: its three vertices touch an infinite cell
@@ -1559,7 +1539,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_volumes(const double *__restric
         }
         __syncthreads();
         for (int i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
-            int k = keys[i];
+            const int k = keys[i];
             if (k >= 0) atomicAdd(W + k, vals[i]);
         }
         __syncthreads();
@@ -1779,11 +1759,11 @@ __global__ void k_adj_vertex_rho(const F *__restrict__ w, const F *__restrict__ 
 // kBlock Morton-consecutive tetrahedra, whose vertices repeat more the
 // larger the chunk; a vertex that finds no slot goes straight to HBM.
 #ifndef DTFE_ADJ_CHUNK
-#define DTFE_ADJ_CHUNK 16
+#define DTFE_ADJ_CHUNK 8
 #endif
 constexpr int kAdjChunk = DTFE_ADJ_CHUNK;
 #ifndef DTFE_ADJ_SLOTS
-#define DTFE_ADJ_SLOTS 2048
+#define DTFE_ADJ_SLOTS 1024
 #endif
 constexpr int kAdjSlots = DTFE_ADJ_SLOTS;
 
@@ -1836,13 +1816,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_tets(const double *__restrict__ 
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kAdjSlots; i += blockDim.x) {
-            const int v = keys[i];
-            if (v >= 0) {
-                atomicAdd(d_pos + 3 * (int64_t)v + 0, (F)vals[i][0]);
-                atomicAdd(d_pos + 3 * (int64_t)v + 1, (F)vals[i][1]);
-                atomicAdd(d_pos + 3 * (int64_t)v + 2, (F)vals[i][2]);
-            }
+        // lane = (slot, component): a vertex's three atomics share a cache line
+        for (int e = threadIdx.x; e < 3 * kAdjSlots; e += blockDim.x) {
+            const int sl = e / 3, c = e - 3 * sl;
+            const int v = keys[sl];
+            if (v >= 0) atomicAdd(d_pos + 3 * (int64_t)v + c, (F)vals[sl][c]);
         }
         __syncthreads();
     }
