@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint / forward ratio")
+    ap.add_argument("--no-method1", action="store_true", help="skip the averaging-method-1 timing")
     ap.add_argument("--adjoint-grid", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -218,6 +219,24 @@ def main():
 
     fe = front_end(ps, prep)
 
+    # averaging method 1 (quasi-random points inside the tetrahedra) on the
+    # same mesh, grid and fields: reported beside the headline, not in it
+    m1 = None
+    if not args.no_method1:
+        rho = api.vertex_density(dm, w, prep.rho_bar)
+        for _ in range(2):
+            api.sample(dm, rho, vel, None, FIELDS, spec, method=1, samples=args.samples)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k1 = []
+        for _ in range(3):
+            api.sample(dm, rho, vel, None, FIELDS, spec, method=1, samples=args.samples)
+            k1.append(ctx.last_timings()[0])
+        torch.cuda.synchronize()
+        m1 = {"ms_per_call": (time.perf_counter() - t0) * 1e3 / 3, "kernel_ms": float(np.mean(k1)),
+              "points_per_grid_cell": args.samples,
+              "workload": "same mesh, grid and fields; averaging method 1 (Sobol points in the tetrahedra)"}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, ps, prep, m, box, n, mc_seed)
@@ -255,6 +274,7 @@ def main():
                      "kernel_ms": k_sample_ms},
         "fp64_path": fp64,
         "front_end_ms": fe,
+        "method1": m1,
         "fwd_adjoint_ratio": None if fa is None else fa["ratio"],
         "fwd_adjoint": fa,
         "cpu_baseline": cpu,

@@ -7,7 +7,7 @@
 // One partition: the GPU replaces the reference's OpenMP split
 // (src/DTFE.cpp:287-385), so results equal the reference's single-thread run.
 // Not provided: --poisson / --randomSample input generation (src/random.cc),
-// --partition serial chunking, averaging method 1 (GSL Sobol), NGP and SPH;
+// --partition serial chunking, NGP and SPH;
 // asking for them is a fatal error, as an invalid option is in the reference.
 #include "DTFE.h"
 
@@ -421,13 +421,20 @@ void dtfe_interpolation(std::vector<Particle_data> *p, std::vector<Sample_point>
         sample_into(ctx, mesh, vf, o.uField, smp, mc, flags, n_rows, uQ);
     }
     if (o.aField.selected()) {
-        if (o.method == 1)
-            fatal("volume averaging method 1 (quasi-random Sobol samples) is not provided by the MI355X drop-in; "
-                  "use method 2 or 3");
         if (o.method == 3 && smp.kind != DTFE_SAMPLE_GRID)
             fatal("averaging method 3 is only available on a regular grid");
-        dtfe_mc mc{o.method == 2 ? DTFE_METHOD_MC : DTFE_METHOD_EQUI, o.noPoints, (uint64_t)o.randomSeed, o.partNo, 0};
-        sample_into(ctx, mesh, vf, o.aField, smp, mc, flags, n_rows, aQ);
+        Field af = o.aField;
+        if (o.method == 1 && af.velocity_std) {
+            // as the reference (averaged_interpolation_1.cc:244-248): a warning, no standard deviation
+            dtfe_dropin::warn(o, "You cannot use the averaging method 1 (Monte Carlo sampling inside the Delaunay "
+                                 "cell) to compute the velocity standard deviation inside the grid cell. Please use "
+                                 "the averaging method 2 '--method 2' (Monte Carlo sampling inside the grid cell) to "
+                                 "compute the velocity standard deviation.");
+            af.velocity_std = false;
+        }
+        const int method = o.method == 1 ? DTFE_METHOD_TET : o.method == 2 ? DTFE_METHOD_MC : DTFE_METHOD_EQUI;
+        dtfe_mc mc{method, o.noPoints, (uint64_t)o.randomSeed, o.partNo, 0};
+        sample_into(ctx, mesh, vf, af, smp, mc, flags, n_rows, aQ);
     }
 }
 

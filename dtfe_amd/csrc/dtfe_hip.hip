@@ -34,6 +34,7 @@
 #include "fastdiv.h"
 #include "mt19937.h"
 #include "predicates.h"
+#include "sobol.h"
 
 namespace dtfe {
 
@@ -1827,6 +1828,309 @@ __global__ void __launch_bounds__(kBlock) k_adj_tets(const double *__restrict__ 
 }
 
 // ----------------------------------------------------------------------------
+// averaging method 1: quasi-random points inside the tetrahedra
+// (interpolateGrid_averaged_1, averaged_interpolation_1.cc:196-421)
+// ----------------------------------------------------------------------------
+// Every tetrahedron whose bounding box meets the region either lies inside
+// one grid cell and adds mean(vertex values) * volume to it, or takes n
+// quasi-random points (n from its volume in grid cells) and adds
+// value(point) * volume / n to the grid cell of each point. Per-cell sums
+// are packed K = 13 + 4 S values per grid cell (density, velocity, velocity
+// gradient, scalars, scalar gradients) and merged per workgroup in an LDS
+// hash table before the HBM atomics; k_method1_finish divides by the grid
+// cell volume and writes the rows with the derived velocity fields.
+struct Method1Args {
+    const double *pos;
+    const int4 *cells;
+    int64_t nc;
+    VertexFieldsDev vf;
+    double box[6];
+    double dx[3];
+    int n[3];
+    double grid_volume;
+    int64_t NN, minNN, maxNN;
+    const double *qr;  // [maxNN][3] quasi-random points of the unit simplex
+    unsigned fields;
+    bool value_bug;
+    void *acc;         // [ncell][K]
+    int K;             // 13 + 4 S
+};
+
+constexpr int kM1Coop = 64;  // tetrahedra with more points go to the whole workgroup
+
+// What one tetrahedron does (checkCellPosition, :87-122, and the point count, :361-364).
+struct M1Tet {
+    int kind;          // 0 outside the region, 1 one grid cell, 2 quasi-random points
+    int64_t cell;      // kind 1: its grid cell
+    int64_t np;        // kind 2: points
+    int4 v;
+    double base[3];    // base vertex position relative to the region corner
+    double vM[3][3];   // vertex k+1 - base
+    double vol;
+    double Ai[3][3];
+};
+
+__device__ __forceinline__ M1Tet m1_tet(const Method1Args &a, int64_t t) {
+    M1Tet T;
+    T.v = a.cells[t];
+    const int id[4] = {T.v.x, T.v.y, T.v.z, T.v.w};
+    d3 X[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) X[k] = load_pos(a.pos, id[k]);
+    const double xs[4][3] = {{X[0].x, X[0].y, X[0].z}, {X[1].x, X[1].y, X[1].z}, {X[2].x, X[2].y, X[2].z},
+                             {X[3].x, X[3].y, X[3].z}};
+    T.kind = 0;
+    T.cell = -1;
+    T.np = 0;
+    // cellOutsideRegion (:72-80): bounding boxes that do not meet (closed)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double lo = fmin(fmin(xs[0][i], xs[1][i]), fmin(xs[2][i], xs[3][i]));
+        const double hi = fmax(fmax(xs[0][i], xs[1][i]), fmax(xs[2][i], xs[3][i]));
+        if (hi < a.box[2 * i] || a.box[2 * i + 1] < lo) return T;
+    }
+    int64_t g[3];
+    bool one = true;
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            T.base[i] = xs[0][i] - a.box[2 * i];
+            g[i] = (int64_t)(int)floor(T.base[i] / a.dx[i]);
+            const double lo = (double)g[i] * a.dx[i] + a.box[2 * i];
+            const double hi = lo + a.dx[i];
+#pragma unroll
+            for (int k = 1; k < 4; ++k) one = one && xs[k][i] >= lo && xs[k][i] <= hi;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) T.vM[k][i] = xs[k + 1][i] - xs[0][i];
+        const double det = T.vM[0][0] * (T.vM[1][1] * T.vM[2][2] - T.vM[1][2] * T.vM[2][1]) -
+                           T.vM[0][1] * (T.vM[1][0] * T.vM[2][2] - T.vM[1][2] * T.vM[2][0]) +
+                           T.vM[0][2] * (T.vM[1][0] * T.vM[2][1] - T.vM[1][1] * T.vM[2][0]);
+        T.vol = det / 6.0;
+    }
+    ref_inverse(X, T.Ai);
+    const bool in_grid = g[0] >= 0 && g[0] < a.n[0] && g[1] >= 0 && g[1] < a.n[1] && g[2] >= 0 && g[2] < a.n[2];
+    if (one && in_grid) {
+        T.kind = 1;
+        T.cell = (g[0] * a.n[1] + g[1]) * a.n[2] + g[2];
+        return T;
+    }
+    T.kind = 2;
+    const double ratio = T.vol / a.grid_volume;
+    const int64_t temp = (int64_t)((double)a.NN * T.vol / a.grid_volume) + 1;
+    T.np = ratio > 1.0 / 6.0 ? (temp > a.maxNN ? a.maxNN : temp) : a.minNN;
+    return T;
+}
+
+// Grid cell of quasi-random point i of T (-1 outside the grid) and its
+// position relative to the base vertex.
+__device__ __forceinline__ int64_t m1_point(const Method1Args &a, const M1Tet &T, int64_t i, double rel[3]) {
+#pragma clang fp contract(off)
+    const double q0 = a.qr[3 * i], q1 = a.qr[3 * i + 1], q2 = a.qr[3 * i + 2];
+    int64_t g[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        rel[d] = q0 * T.vM[0][d] + q1 * T.vM[1][d] + q2 * T.vM[2][d];
+        g[d] = (int64_t)(int)floor((T.base[d] + rel[d]) / a.dx[d]);
+        if (g[d] < 0 || g[d] >= a.n[d]) return -1;
+    }
+    return (g[0] * a.n[1] + g[1]) * a.n[2] + g[2];
+}
+
+// The per-cell sums of one workgroup: LDS slots keyed by grid cell, K FP64
+// values each; a cell that finds no slot goes straight to HBM.
+template <typename F, int K>
+struct M1Sink {
+    int64_t *keys;   // [S] -1 = free
+    double *vals;    // [S][K]
+    int slots;
+    __device__ int find(int64_t cell) const {
+        unsigned h = (unsigned)(((uint64_t)cell * 0x9E3779B97F4A7C15ull) >> 40) & (unsigned)(slots - 1);
+        for (int probe = 0; probe < 32; ++probe) {
+            const unsigned long long k =
+                atomicCAS((unsigned long long *)&keys[h], (unsigned long long)-1, (unsigned long long)cell);
+            if ((long long)k == -1 || (int64_t)k == cell) return (int)h;
+            h = (h + 1) & (unsigned)(slots - 1);
+        }
+        return -1;
+    }
+    __device__ void add(const Method1Args &a, int64_t cell, int slot, int c, double v) const {
+        if (v == 0.0) return;
+        if (slot >= 0) atomicAdd(&vals[slot * K + c], v);
+        else atomic_add_f<F>(a.acc, cell * a.K + c, (F)v);
+    }
+};
+
+// Adds one contribution (mean of the vertex values when rel is null) with
+// weight w: density, velocity and velocity gradient (G constant per
+// tetrahedron), scalars and their gradients, in the reference's formulas.
+template <typename F, bool VEL, int NS, int K>
+__device__ __forceinline__ void m1_add(const Method1Args &a, const M1Tet &T, const double *rel, double w,
+                                       int64_t cell, const M1Sink<F, K> &sink) {
+    const int slot = sink.find(cell);
+    const int id[4] = {T.v.x, T.v.y, T.v.z, T.v.w};
+    F AiF[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) AiF[r][c] = (F)T.Ai[r][c];
+    F rl[3] = {0, 0, 0};
+    if (rel)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) rl[d] = (F)rel[d];
+    if (a.fields & DTFE_FIELD_DENSITY) {
+        F r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = ldf<F>(a.vf.rho, id[k]);
+        F val;
+        if (!rel) {
+            val = (((r[0] + r[1]) + r[2]) + r[3]) / (F)4;
+        } else {
+            const F dd[3] = {r[1] - r[0], r[2] - r[0], r[3] - r[0]};
+            F g[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g[i] = AiF[i][0] * dd[0] + AiF[i][1] * dd[1] + AiF[i][2] * dd[2];
+            val = r[0] + (rl[0] * g[0] + rl[1] * g[1] + rl[2] * g[2]);
+        }
+        sink.add(a, cell, slot, 0, (double)val * w);
+    }
+    if (VEL && (a.fields & (DTFE_FIELD_VELOCITY | kGradUsers))) {
+        F vb[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vb[k][c] = ldf<F>(a.vf.vel, 3 * (int64_t)id[k] + c);
+        F G[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                G[i][c] = AiF[i][0] * (vb[1][c] - vb[0][c]) + AiF[i][1] * (vb[2][c] - vb[0][c]) +
+                          AiF[i][2] * (vb[3][c] - vb[0][c]);
+        if (a.fields & DTFE_FIELD_VELOCITY)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                F val;
+                if (!rel) val = (((vb[0][c] + vb[1][c]) + vb[2][c]) + vb[3][c]) / (F)4;
+                else if (a.value_bug) val = G[2][c] * rl[2] + vb[0][c];
+                else val = (G[0][c] * rl[0] + G[1][c] * rl[1] + G[2][c] * rl[2]) + vb[0][c];
+                sink.add(a, cell, slot, 1 + c, (double)val * w);
+            }
+        if (a.fields & kGradUsers)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) sink.add(a, cell, slot, 4 + c * 3 + i, (double)G[i][c] * w);
+    }
+    if (NS > 0 && (a.fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT))) {
+        const int S = a.vf.S;
+        for (int q = 0; q < S; ++q) {
+            F sb[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sb[k] = ldf<F>(a.vf.scal, (int64_t)S * id[k] + q);
+            F g[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                g[i] = AiF[i][0] * (sb[1] - sb[0]) + AiF[i][1] * (sb[2] - sb[0]) + AiF[i][2] * (sb[3] - sb[0]);
+            if (a.fields & DTFE_FIELD_SCALAR) {
+                F val;
+                if (!rel) val = (((sb[0] + sb[1]) + sb[2]) + sb[3]) / (F)4;
+                else if (a.value_bug) val = g[2] * rl[2] + sb[0];
+                else val = (g[0] * rl[0] + g[1] * rl[1] + g[2] * rl[2]) + sb[0];
+                sink.add(a, cell, slot, 13 + q, (double)val * w);
+            }
+            if (a.fields & DTFE_FIELD_SCALAR_GRADIENT)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) sink.add(a, cell, slot, 13 + S + 3 * q + i, (double)g[i] * w);
+        }
+    }
+}
+
+// One tile of kBlock consecutive (Morton-ordered) tetrahedra per pass: each
+// thread adds its own tetrahedron; those with more than kM1Coop points are
+// queued and their points shared by the whole workgroup; then the merged
+// cells go to HBM, one value per lane.
+template <typename F, bool VEL, int NS>
+__global__ void __launch_bounds__(kBlock) k_method1(Method1Args a) {
+    constexpr int K = 13 + 4 * NS;
+    constexpr int kSlots = NS > 0 ? 256 : 512;
+    __shared__ int64_t keys[kSlots];
+    __shared__ double vals[kSlots * K];
+    __shared__ int64_t queue[kBlock];
+    __shared__ int nq;
+    const M1Sink<F, K> sink{keys, vals, kSlots};
+    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < a.nc; t0 += (int64_t)gridDim.x * kBlock) {
+        for (int i = threadIdx.x; i < kSlots; i += kBlock) keys[i] = -1;
+        for (int i = threadIdx.x; i < kSlots * K; i += kBlock) vals[i] = 0.0;
+        if (threadIdx.x == 0) nq = 0;
+        __syncthreads();
+        const int64_t t = t0 + threadIdx.x;
+        if (t < a.nc) {
+            const M1Tet T = m1_tet(a, t);
+            if (T.kind == 1) {
+                m1_add<F, VEL, NS, K>(a, T, nullptr, T.vol, T.cell, sink);
+            } else if (T.kind == 2) {
+                if (T.np > kM1Coop) {
+                    queue[atomicAdd(&nq, 1)] = t;
+                } else {
+                    const double w = T.vol / (double)T.np;
+                    for (int64_t i = 0; i < T.np; ++i) {
+                        double rel[3];
+                        const int64_t cell = m1_point(a, T, i, rel);
+                        if (cell >= 0) m1_add<F, VEL, NS, K>(a, T, rel, w, cell, sink);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const int n_queued = nq;
+        for (int k = 0; k < n_queued; ++k) {
+            const M1Tet T = m1_tet(a, queue[k]);
+            const double w = T.vol / (double)T.np;
+            for (int64_t i = threadIdx.x; i < T.np; i += kBlock) {
+                double rel[3];
+                const int64_t cell = m1_point(a, T, i, rel);
+                if (cell >= 0) m1_add<F, VEL, NS, K>(a, T, rel, w, cell, sink);
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < kSlots * K; e += kBlock) {
+            const int sl = e / K, c = e - sl * K;
+            const int64_t cell = keys[sl];
+            const double v = vals[e];
+            if (cell >= 0 && c < a.K && v != 0.0) atomic_add_f<F>(a.acc, cell * a.K + c, (F)v);
+        }
+        __syncthreads();
+    }
+}
+
+// Per-cell sums / grid cell volume -> output rows (with divergence, shear
+// and vorticity of the averaged gradient).
+template <typename F, bool VEL, int NS>
+__global__ void __launch_bounds__(kBlock) k_method1_finish(const F *__restrict__ acc, int64_t ncell, int K, int S,
+                                                           unsigned fields, F scale, OutPtrs out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += stride) {
+        const F *r = acc + c * K;
+        RowAcc<F, NS> ra;
+        ra.clear();
+        ra.d = r[0];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ra.v[i] = r[1 + i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ra.vg[i] = r[4 + i];
+        if (NS > 0) {
+            for (int q = 0; q < S; ++q) ra.s[q] = r[13 + q];
+            for (int q = 0; q < 3 * S; ++q) ra.sg[q] = r[13 + S + q];
+        }
+        store_row<F, VEL, NS>(out, c, ra, scale, fields, S);
+    }
+}
+
+// ----------------------------------------------------------------------------
 // CIC / TSC deposits
 // ----------------------------------------------------------------------------
 struct DepositArgs {
@@ -2454,6 +2758,92 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     return DTFE_OK;
 }
 
+// Averaging method 1 on the regular grid of `a` (rows = grid cells).
+int launch_method1(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *vf, unsigned fields, int S,
+                   const double box[6], int64_t NN, unsigned flags, const SampleArgs &a) {
+    int rc;
+    const bool f64 = vf->dtype == DTFE_F64;
+    const size_t esz = f64 ? 8 : 4;
+    const int K = 13 + 4 * S;
+    const int64_t ncell = a.n_rows;
+    const int64_t maxNN = 100 * NN;
+    if (ctx->m1_points != maxNN) {
+        const std::vector<double> q = sobol_simplex(maxNN);
+        if ((rc = ensure(ctx, ctx->m1_qr, q.size() * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpy(ctx->m1_qr.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+        ctx->m1_points = maxNN;
+    }
+    if ((rc = ensure(ctx, ctx->m1_acc, (size_t)ncell * K * esz))) return rc;
+    if (f64)
+        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(ncell * K)), dim3(kBlock), 0, ctx->stream,
+                           (double *)ctx->m1_acc.p, ncell * K, 0.0);
+    else
+        hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(ncell * K)), dim3(kBlock), 0, ctx->stream,
+                           (float *)ctx->m1_acc.p, ncell * K, 0.0f);
+    Method1Args ma{};
+    ma.pos = m->pos;
+    ma.cells = (const int4 *)m->cells;
+    ma.nc = m->n_cells;
+    ma.vf = a.vf;
+    for (int i = 0; i < 6; ++i) ma.box[i] = box[i];
+    for (int i = 0; i < 3; ++i) {
+        ma.dx[i] = a.dx[i];
+        ma.n[i] = (int)a.n[i];
+    }
+    ma.grid_volume = a.dx[0] * a.dx[1] * a.dx[2];
+    ma.NN = NN;
+    ma.minNN = NN / 6 + 1;
+    ma.maxNN = maxNN;
+    ma.qr = (const double *)ctx->m1_qr.p;
+    ma.fields = fields;
+    ma.value_bug = (flags & DTFE_COMPAT_VALUE_BUG) != 0;
+    ma.acc = ctx->m1_acc.p;
+    ma.K = K;
+    const bool vel = (fields & (DTFE_FIELD_VELOCITY | kGradUsers)) != 0;
+    const dim3 g(grid_for(m->n_cells));
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (f64) {
+        if (vel) {
+            if (S) hipLaunchKernelGGL((k_method1<double, true, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
+            else hipLaunchKernelGGL((k_method1<double, true, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+        } else {
+            if (S) hipLaunchKernelGGL((k_method1<double, false, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
+            else hipLaunchKernelGGL((k_method1<double, false, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+        }
+    } else {
+        if (vel) {
+            if (S) hipLaunchKernelGGL((k_method1<float, true, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
+            else hipLaunchKernelGGL((k_method1<float, true, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+        } else {
+            if (S) hipLaunchKernelGGL((k_method1<float, false, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
+            else hipLaunchKernelGGL((k_method1<float, false, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+        }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    const dim3 gf(grid_for(ncell));
+    if (f64) {
+        const double scale = 1.0 / ma.grid_volume;
+        if (S) hipLaunchKernelGGL((k_method1_finish<double, true, kMaxScalars>), gf, dim3(kBlock), 0, ctx->stream,
+                                  (const double *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
+        else hipLaunchKernelGGL((k_method1_finish<double, true, 0>), gf, dim3(kBlock), 0, ctx->stream,
+                                (const double *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
+    } else {
+        const float scale = (float)(1.0 / ma.grid_volume);
+        if (S) hipLaunchKernelGGL((k_method1_finish<float, true, kMaxScalars>), gf, dim3(kBlock), 0, ctx->stream,
+                                  (const float *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
+        else hipLaunchKernelGGL((k_method1_finish<float, true, 0>), gf, dim3(kBlock), 0, ctx->stream,
+                                (const float *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->last_ms[0] = ms;
+    ctx->last_ms[1] = ctx->last_ms[2] = ctx->last_ms[4] = 0.0;
+    ctx->last_deferred = 0;
+    return DTFE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2490,7 +2880,8 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_b, &ctx->bbox_part,
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
-                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->front};
+                      &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->m1_qr,
+                      &ctx->m1_acc, &ctx->front};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 9; ++i)
@@ -2607,8 +2998,18 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
     for (auto &r : req)
         if ((fields & r.bit) && !r.p) return fail(ctx, DTFE_ERR_ARGUMENT, std::string("output for ") + r.name + " is NULL");
     const int method = mc->method;
-    if (method != DTFE_METHOD_CENTRE && method != DTFE_METHOD_MC && method != DTFE_METHOD_EQUI)
-        return fail(ctx, DTFE_ERR_ARGUMENT, "method must be 0 (centre), 2 (Monte Carlo) or 3 (equidistant)");
+    if (method != DTFE_METHOD_CENTRE && method != DTFE_METHOD_TET && method != DTFE_METHOD_MC &&
+        method != DTFE_METHOD_EQUI)
+        return fail(ctx, DTFE_ERR_ARGUMENT,
+                    "method must be 0 (centre), 1 (points in the tetrahedra), 2 (Monte Carlo) or 3 (equidistant)");
+    if (method == DTFE_METHOD_TET) {
+        if (smp->kind != DTFE_SAMPLE_GRID)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "averaging method 1 is only available on a regular grid");
+        if (fields & DTFE_FIELD_VELOCITY_STD)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "averaging method 1 does not give the velocity standard deviation");
+        if (out->located_cell)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "averaging method 1 has no located cell per row");
+    }
     if (method != DTFE_METHOD_CENTRE && (mc->samples < 1 || mc->samples > kMaxSamples))
         return fail(ctx, DTFE_ERR_ARGUMENT, "samples per cell must be in [1, " + std::to_string(kMaxSamples) + "]");
     if (method == DTFE_METHOD_EQUI && smp->kind != DTFE_SAMPLE_GRID)
@@ -2665,9 +3066,10 @@ int dtfe_hip_sample(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields 
                                out->located_cell, n_rows, -1);
         return fail(ctx, DTFE_ERR_TOO_FEW_VERTICES, "fewer than 4 vertices: fields are zero");
     }
+    const int ns = want_scal ? vf->n_scalar : 0;
+    if (method == DTFE_METHOD_TET) return launch_method1(ctx, m, vf, fields, ns, smp->box, mc->samples, flags, a);
     if ((rc = use_locator(ctx, m, a))) return rc;
     if ((rc = setup_tables(ctx, m, mc, a, axes, trig))) return rc;
-    const int ns = want_scal ? vf->n_scalar : 0;
     return launch_rows(ctx, a, mc, vf->dtype == DTFE_F64, want_vel, ns, false);
 }
 

@@ -68,6 +68,8 @@ extern "C" {
 #define DTFE_SAMPLE_POINTS 2   /* user points with per-point cell sizes (delta) */
 
 #define DTFE_METHOD_CENTRE 0   /* value at the sample point (the reference's uField) */
+#define DTFE_METHOD_TET    1   /* averaging method 1: quasi-random points in the tetrahedra (grid only;
+                                  samples = points per grid cell on average) */
 #define DTFE_METHOD_MC     2   /* Monte-Carlo average over the cell (averaging method 2) */
 #define DTFE_METHOD_EQUI   3   /* equidistant n^3 average (averaging method 3, grid only) */
 

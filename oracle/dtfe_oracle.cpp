@@ -543,6 +543,81 @@ int64_t oracle_sample(int64_t nv, const double *pos, int64_t nc, const int32_t *
     return failed ? -1 : o;
 }
 
+// ---- averaging method 1 (averaged_interpolation_1.cc) ---------------------------------
+// GSL's Sobol generator in 3 dimensions (gsl_qrng_sobol; GSL is not vendored
+// by the reference and is absent here, so the direction numbers below are a
+// restatement of GSL's qrng/sobol.c: Bratley & Fox, ACM TOMS 14 (1988),
+// Algorithm 659, with 30-bit numerators). Dimension 0 is the van der Corput
+// sequence; dimension 1 uses the primitive polynomial x + 1 with initial
+// value 1; dimension 2 uses x^2 + x + 1 with initial values 1, 1. Each call
+// returns the next point; the first is (1/2, 1/2, 1/2).
+struct Sobol3 {
+    static constexpr int kBits = 30;
+    uint32_t v[kBits][3];
+    uint32_t last[3] = {0, 0, 0};
+    uint64_t count = 0;
+    double inv;
+    Sobol3() {
+        const int poly[3] = {1, 3, 7}, degree[3] = {0, 1, 2};
+        const int init[2][3] = {{0, 1, 1}, {0, 0, 1}};
+        for (int k = 0; k < kBits; ++k) v[k][0] = 1;
+        for (int d = 1; d < 3; ++d) {
+            const int deg = degree[d];
+            int inc[8];
+            int pbits = poly[d];
+            for (int k = deg - 1; k >= 0; --k) {
+                inc[k] = (pbits % 2) == 1;
+                pbits /= 2;
+            }
+            for (int j = 0; j < deg; ++j) v[j][d] = (uint32_t)init[j][d];
+            for (int j = deg; j < kBits; ++j) {
+                uint32_t nv = v[j - deg][d];
+                uint32_t ell = 1;
+                for (int k = 0; k < deg; ++k) {
+                    ell *= 2;
+                    if (inc[k]) nv ^= ell * v[j - k - 1][d];
+                }
+                v[j][d] = nv;
+            }
+        }
+        uint32_t ell = 1;
+        for (int j = kBits - 2; j >= 0; --j) {
+            ell *= 2;
+            for (int d = 0; d < 3; ++d) v[j][d] *= ell;
+        }
+        inv = 1.0 / (2.0 * ell);
+    }
+    void next(double out[3]) {
+        int ell = 0;
+        uint64_t c = count;
+        while (true) {
+            ++ell;
+            if (c % 2 == 1) c /= 2;
+            else break;
+        }
+        for (int d = 0; d < 3; ++d) {
+            last[d] ^= v[ell - 1][d];
+            out[d] = last[d] * inv;
+        }
+        ++count;
+    }
+};
+
+// quasiRandomSequence (averaged_interpolation_1.cc:31-48): the first n Sobol
+// points whose coordinates sum to at most 1.
+static void sobol_simplex(int64_t n, double *out) {
+    Sobol3 q;
+    for (int64_t i = 0; i < n;) {
+        double r[3];
+        q.next(r);
+        double sum = r[0] + r[1] + r[2];
+        if (sum <= 1.) {
+            for (int j = 0; j < 3; ++j) out[3 * i + j] = r[j];
+            ++i;
+        }
+    }
+}
+
 // ---- derived velocity fields (DTFE.cpp:394-433) -------------------------------------
 void oracle_div_shear_vort(int64_t n, const double *grad, double *div, double *shear, double *vort) {
     for (int64_t c = 0; c < n; ++c) {
@@ -700,6 +775,200 @@ int64_t oracle_periodic_images(int64_t np, const double *pos, const double *full
                 }
             }
     return k;
+}
+
+void oracle_sobol_raw(int64_t n, double *out) {
+    Sobol3 q;
+    for (int64_t i = 0; i < n; ++i) q.next(out + 3 * i);
+}
+
+void oracle_sobol_simplex(int64_t n, double *out) { sobol_simplex(n, out); }
+
+// interpolateGrid_averaged_1 (averaged_interpolation_1.cc:196-421): every
+// finite tetrahedron whose bounding box meets the region adds its field
+// integral to the grid cells: a tetrahedron inside one grid cell adds
+// mean(vertex values) * volume (checkCellPosition :87-122); any other takes
+// n quasi-random points x_base + q vertexMatrix (n from its volume in grid
+// cells: NN per grid cell, between NN/6 + 1 and 100 NN) and adds
+// value(point) * volume / n to the grid cell of each point inside the region.
+// The sums are divided by the grid cell volume. Returns -1 for unsupported
+// fields (the velocity standard deviation) or fewer than 4 vertices' zero fill
+// is done by the caller.
+int64_t oracle_grid_method1(int64_t nv, const double *pos, int64_t nc, const int32_t *cells, const double *rho,
+                            const double *vel, const double *scalar, int S, const double *box, const int64_t *n,
+                            int64_t NN, int value_bug, const int *fields, double *o_density, double *o_velocity,
+                            double *o_velgrad, double *o_scalar, double *o_scalargrad) {
+    FieldReq f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
+    if (f.velocity_std) return -1;
+    const int64_t ncell = n[0] * n[1] * n[2];
+    auto zero = [&](double *o, int64_t k) {
+        if (o) std::fill(o, o + k * ncell, 0.0);
+    };
+    zero(o_density, 1);
+    zero(o_velocity, 3);
+    zero(o_velgrad, 9);
+    zero(o_scalar, S);
+    zero(o_scalargrad, 3 * S);
+    if (nv < 4) return 0;
+    const int64_t minNN = NN / 6 + 1;
+    const double minRatio = 1. / 6.;
+    const int64_t maxNN = 100 * NN;
+    double dx[3];
+    for (int i = 0; i < 3; ++i) dx[i] = (box[2 * i + 1] - box[2 * i]) / (double)n[i];
+    const double gridCellVolume = dx[0] * dx[1] * dx[2];
+    std::vector<double> qr(3 * maxNN);
+    sobol_simplex(maxNN, qr.data());
+    auto add = [&](int64_t idx, int64_t c, const double *rel, double weight, bool mean, const double Ai[3][3]) {
+        const int32_t *cv = cells + 4 * c;
+        if (f.density && o_density) {
+            if (mean) {
+                double t = 0.;
+                for (int k = 0; k < 4; ++k) t += rho[cv[k]];
+                o_density[idx] += t / 4. * weight;
+            } else {
+                double dd[3], g[3];
+                for (int i = 0; i < 3; ++i) dd[i] = rho[cv[i + 1]] - rho[cv[0]];
+                for (int i = 0; i < 3; ++i) {
+                    g[i] = 0.;
+                    for (int j = 0; j < 3; ++j) g[i] += Ai[i][j] * dd[j];
+                }
+                double t = rel[0] * g[0] + rel[1] * g[1] + rel[2] * g[2];
+                o_density[idx] += (rho[cv[0]] + t) * weight;
+            }
+        }
+        if ((f.velocity && o_velocity) || (f.velocity_gradient && o_velgrad)) {
+            const double *b = vel + 3 * (int64_t)cv[0];
+            double T[3][3], G[3][3];
+            for (int v = 0; v < 3; ++v)
+                for (int k = 0; k < 3; ++k) T[v][k] = vel[3 * (int64_t)cv[v + 1] + k] - b[k];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    G[i][j] = 0.;
+                    for (int k = 0; k < 3; ++k) G[i][j] += Ai[i][k] * T[k][j];
+                }
+            if (f.velocity && o_velocity)
+                for (int i = 0; i < 3; ++i) {
+                    double val;
+                    if (mean) {
+                        double t = 0.;
+                        for (int k = 0; k < 4; ++k) t += vel[3 * (int64_t)cv[k] + i];
+                        val = t / 4.;
+                    } else {
+                        double t = 0.;
+                        for (int j = 0; j < 3; ++j) {
+                            if (value_bug) t = G[j][i] * rel[j];
+                            else t += G[j][i] * rel[j];
+                        }
+                        val = t + b[i];
+                    }
+                    o_velocity[3 * idx + i] += val * weight;
+                }
+            if (f.velocity_gradient && o_velgrad)
+                for (int j = 0; j < 3; ++j)
+                    for (int i = 0; i < 3; ++i) o_velgrad[9 * idx + j * 3 + i] += G[i][j] * weight;
+        }
+        if ((f.scalar && o_scalar) || (f.scalar_gradient && o_scalargrad)) {
+            const double *b = scalar + (int64_t)S * cv[0];
+            std::vector<double> T(3 * S), G(3 * S);
+            for (int v = 0; v < 3; ++v)
+                for (int k = 0; k < S; ++k) T[v * S + k] = scalar[(int64_t)S * cv[v + 1] + k] - b[k];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < S; ++j) {
+                    double acc = 0.;
+                    for (int k = 0; k < 3; ++k) acc += Ai[i][k] * T[k * S + j];
+                    G[i * S + j] = acc;
+                }
+            if (f.scalar && o_scalar)
+                for (int i = 0; i < S; ++i) {
+                    double val;
+                    if (mean) {
+                        double t = 0.;
+                        for (int k = 0; k < 4; ++k) t += scalar[(int64_t)S * cv[k] + i];
+                        val = t / 4.;
+                    } else {
+                        double t = 0.;
+                        for (int j = 0; j < 3; ++j) {
+                            if (value_bug) t = G[j * S + i] * rel[j];
+                            else t += G[j * S + i] * rel[j];
+                        }
+                        val = t + b[i];
+                    }
+                    o_scalar[(int64_t)S * idx + i] += val * weight;
+                }
+            if (f.scalar_gradient && o_scalargrad)
+                for (int j = 0; j < S; ++j)
+                    for (int i = 0; i < 3; ++i) o_scalargrad[3 * S * idx + j * 3 + i] += G[i * S + j] * weight;
+        }
+    };
+    auto cell_index = [&](const int64_t g[3], int64_t *idx) {
+        for (int i = 0; i < 3; ++i)
+            if (g[i] < 0 || g[i] >= n[i]) return false;
+        *idx = (g[0] * n[1] + g[1]) * n[2] + g[2];
+        return true;
+    };
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t *cv = cells + 4 * c;
+        const double *X[4] = {pos + 3 * (int64_t)cv[0], pos + 3 * (int64_t)cv[1], pos + 3 * (int64_t)cv[2],
+                              pos + 3 * (int64_t)cv[3]};
+        // cellOutsideRegion (:72-80): bounding boxes that do not meet (closed)
+        bool outside = false;
+        for (int i = 0; i < 3; ++i) {
+            double lo = X[0][i], hi = X[0][i];
+            for (int k = 1; k < 4; ++k) lo = std::min(lo, X[k][i]), hi = std::max(hi, X[k][i]);
+            if (hi < box[2 * i] || box[2 * i + 1] < lo) outside = true;
+        }
+        if (outside) continue;
+        // checkCellPosition (:87-122)
+        double basePosition[3], cellLo[3], cellHi[3];
+        int64_t baseCell[3];
+        for (int i = 0; i < 3; ++i) {
+            basePosition[i] = X[0][i] - box[2 * i];
+            baseCell[i] = (int64_t)(int)std::floor(basePosition[i] / dx[i]);
+            cellLo[i] = baseCell[i] * dx[i] + box[2 * i];
+            cellHi[i] = cellLo[i] + dx[i];
+        }
+        bool one = true;
+        for (int k = 1; k < 4 && one; ++k)
+            for (int i = 0; i < 3; ++i)
+                if (!(X[k][i] >= cellLo[i] && X[k][i] <= cellHi[i])) one = false;
+        double vM[3][3];
+        for (int v = 0; v < 3; ++v)
+            for (int i = 0; i < 3; ++i) vM[v][i] = X[v + 1][i] - X[0][i];
+        // volume (triangulation_miscellaneous.cc:204-213)
+        double det = vM[0][0] * (vM[1][1] * vM[2][2] - vM[1][2] * vM[2][1]) -
+                     vM[0][1] * (vM[1][0] * vM[2][2] - vM[1][2] * vM[2][0]) +
+                     vM[0][2] * (vM[1][0] * vM[2][1] - vM[1][1] * vM[2][0]);
+        const double cellVolume = det / 6.0;
+        double Ai[3][3];
+        matrixInverse(vM, Ai);
+        int64_t idx;
+        if (one && cell_index(baseCell, &idx)) {
+            add(idx, c, nullptr, cellVolume, true, Ai);
+            continue;
+        }
+        const int64_t tempInt = (int64_t)(NN * cellVolume / gridCellVolume) + 1;
+        const int64_t np = (cellVolume / gridCellVolume > minRatio) ? (tempInt > maxNN ? maxNN : tempInt) : minNN;
+        const double factor = cellVolume / (double)np;
+        for (int64_t i = 0; i < np; ++i) {
+            const double *q = qr.data() + 3 * i;
+            double rel[3];
+            for (int d = 0; d < 3; ++d) rel[d] = q[0] * vM[0][d] + q[1] * vM[1][d] + q[2] * vM[2][d];
+            int64_t g[3];
+            for (int d = 0; d < 3; ++d) g[d] = (int64_t)(int)std::floor((basePosition[d] + rel[d]) / dx[d]);
+            if (!cell_index(g, &idx)) continue;
+            add(idx, c, rel, factor, false, Ai);
+        }
+    }
+    auto scale = [&](double *o, int64_t k) {
+        if (o)
+            for (int64_t i = 0; i < k * ncell; ++i) o[i] /= gridCellVolume;
+    };
+    scale(o_density, 1);
+    scale(o_velocity, 3);
+    scale(o_velgrad, 9);
+    scale(o_scalar, S);
+    scale(o_scalargrad, 3 * S);
+    return ncell;
 }
 
 double oracle_average_density(int64_t np, const double *weight) {

@@ -45,6 +45,10 @@ def load():
                                   i32, i32, C.c_uint64, i32, i32, i32, P, P, P, P, P, P, P, P, P, P]
     lib.oracle_sample.restype = i64
     lib.oracle_div_shear_vort.argtypes = [i64, P, P, P, P]
+    lib.oracle_sobol_raw.argtypes = [i64, P]
+    lib.oracle_sobol_simplex.argtypes = [i64, P]
+    lib.oracle_grid_method1.argtypes = [i64, P, i64, P, P, P, P, i32, P, P, i64, i32, P, P, P, P, P, P]
+    lib.oracle_grid_method1.restype = i64
     lib.oracle_deposit.argtypes = [i32, i64, P, P, P, P, P, d, i32, i32, P, P]
     lib.oracle_periodic_images.argtypes = [i64, P, P, P, i64, P, P]
     lib.oracle_periodic_images.restype = i64
@@ -179,6 +183,54 @@ def sample(mesh, rho, vel=None, scalar=None, fields=("density",), kind="grid", b
             res["velocity_shear"] = sh
         if "velocity_vorticity" in fields:
             res["velocity_vorticity"] = vo
+    return res
+
+
+def sobol_raw(n: int) -> np.ndarray:
+    """The first n points of the 3-dimensional Sobol sequence (GSL's generator, restated)."""
+    out = np.empty((n, 3))
+    load().oracle_sobol_raw(n, _p(out))
+    return out
+
+
+def sobol_simplex(n: int) -> np.ndarray:
+    """quasiRandomSequence: the first n Sobol points with coordinate sum <= 1."""
+    out = np.empty((n, 3))
+    load().oracle_sobol_simplex(n, _p(out))
+    return out
+
+
+def grid_method1(mesh, rho, vel=None, scalar=None, fields=("density",), box=None, n=None, samples=8,
+                 value_bug=True):
+    """interpolateGrid_averaged_1 (averaging method 1) on a regular grid; a
+    dict of float64 arrays (derived velocity fields from the averaged gradient)."""
+    lib = load()
+    pos, cells = _f64(mesh.pos), _i32(mesh.cells)
+    rho, vel, scalar = _f64(rho), _f64(vel), _f64(scalar)
+    S = 0 if scalar is None else scalar.shape[1]
+    nn = np.asarray(n, dtype=np.int64)
+    ncell = int(np.prod(nn))
+    want = {f: (f in fields) for f in FIELDS}
+    grad_needed = [f for f in ("velocity_divergence", "velocity_shear", "velocity_vorticity") if f in fields]
+    if grad_needed:
+        want["velocity_gradient"] = True
+    flags = np.array([int(want[f]) for f in FIELDS], dtype=np.int32)
+    comp = {"density": 1, "velocity": 3, "velocity_gradient": 9, "scalar": S, "scalar_gradient": 3 * S}
+    outs = {f: (np.zeros((ncell, comp[f]) if comp[f] != 1 else ncell) if want.get(f) else None) for f in comp}
+    rc = lib.oracle_grid_method1(pos.shape[0], _p(pos), cells.shape[0], _p(cells), _p(rho), _p(vel), _p(scalar),
+                                 S, _p(_f64(box)), _p(nn), int(samples), int(value_bug), _p(flags),
+                                 _p(outs["density"]), _p(outs["velocity"]), _p(outs["velocity_gradient"]),
+                                 _p(outs["scalar"]), _p(outs["scalar_gradient"]))
+    if rc < 0:
+        raise ValueError("averaging method 1 does not give the velocity standard deviation")
+    res = {f: outs[f] for f in comp if f in fields}
+    if grad_needed:
+        g = np.ascontiguousarray(outs["velocity_gradient"])
+        dv, sh, vo = np.zeros(ncell), np.zeros((ncell, 5)), np.zeros((ncell, 3))
+        lib.oracle_div_shear_vort(ncell, _p(g), _p(dv), _p(sh), _p(vo))
+        for f, v in (("velocity_divergence", dv), ("velocity_shear", sh), ("velocity_vorticity", vo)):
+            if f in fields:
+                res[f] = v
     return res
 
 

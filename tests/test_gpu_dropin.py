@@ -116,3 +116,25 @@ def test_dropin_user_points_and_cic(gpu, tmp_path):
                      M.average_density(ps.weight))
     got = np.fromfile(tmp2 / "out.a.density", dtype=np.float64)
     assert np.max(np.abs(got - d)) / np.max(np.abs(d)) < 1e-10
+
+
+def test_dropin_method1(gpu, tmp_path):
+    """Averaging method 1 through DTFE(): the driver's padding and
+    tessellation, then the tetrahedron points kernel, against the oracle."""
+    prob = P.clustered_problem()
+    ps = prob.ps
+    parts = np.concatenate([ps.pos, ps.weight[:, None], ps.vel], 1)
+    box = " ".join(str(x) for x in ps.box)
+    opts = [f"box {box}", "grid 12 12 12", "periodic 1", "fields_a density,velocity,gradient,divergence",
+            "method 1", "points 8"]
+    out = _run(tmp_path, opts, parts, real="double")
+    assert "particles_left=0" in out
+    rho, _, _ = O.vertex_density(prob.mesh, prob.w, prob.rho_bar)
+    ref = O.grid_method1(prob.mesh, rho, prob.vel, fields=("density", "velocity", "velocity_gradient",
+                                                            "velocity_divergence"),
+                         box=ps.box, n=(12, 12, 12), samples=8)
+    for f, k in (("density", 1), ("velocity", 3), ("velocity_gradient", 9), ("velocity_divergence", 1)):
+        got = _load(tmp_path, "a", f, np.float64, k)
+        err = np.max(np.abs(got - ref[f])) / np.max(np.abs(ref[f]))
+        print(f"drop-in method 1 {f}: {err:.3e}")
+        assert err < 1e-10, f

@@ -366,3 +366,41 @@ def test_deferral_queue_overflow(gpu, method, samples, monkeypatch):
         assert np.array_equal(out["located_cell"].cpu().numpy(), ref["located_cell"])
         for f in fields:
             assert rel_err(out[f].cpu().numpy(), ref[f]) < TOL64, (cap, f)
+
+
+@pytest.mark.parametrize("dtype,tol,value_bug,scalars", [(torch.float64, 1e-10, True, False),
+                                                          (torch.float64, 1e-10, False, True),
+                                                          (torch.float32, 1e-4, True, False)])
+def test_grid_method1(gpu, dtype, tol, value_bug, scalars):
+    """Averaging method 1 (quasi-random points inside the tetrahedra) against
+    the oracle on the same mesh and quasi-random table."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, sc = gpu_setup(prob, dtype)
+    n = (12, 11, 10)
+    fields = ["density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_vorticity"]
+    if scalars:
+        fields += ["scalar", "scalar_gradient"]
+    out = api.sample(dm, rho, vel, sc if scalars else None, fields, api.GridSpec(prob.region, n), method=1,
+                     samples=8, value_bug=value_bug)
+    ref = O.grid_method1(prob.mesh, oracle_rho(prob), prob.vel, prob.scalar if scalars else None, fields=fields,
+                         box=prob.region, n=n, samples=8, value_bug=value_bug)
+    for f in fields:
+        e = rel_err(out[f].cpu().numpy(), ref[f])
+        print(f"method 1 {dtype} bug={value_bug} {f}: {e:.3e}")
+        assert e < tol, f
+
+
+def test_grid_method1_sub_region(gpu):
+    """A region inside the mesh: tetrahedra across its faces keep only their
+    points inside it; grid cells at the faces are partly covered."""
+    from dtfe_amd import api
+    prob = P.clustered_problem()
+    dm, rho, vel, _ = gpu_setup(prob, torch.float64)
+    box = (3.0, 15.5, 2.25, 17.0, 4.0, 12.0)
+    n = (9, 10, 6)
+    out = api.sample(dm, rho, vel, None, ["density", "velocity"], api.GridSpec(box, n), method=1, samples=20)
+    ref = O.grid_method1(prob.mesh, oracle_rho(prob), prob.vel, fields=("density", "velocity"), box=box, n=n,
+                         samples=20)
+    for f in ("density", "velocity"):
+        assert rel_err(out[f].cpu().numpy(), ref[f]) < TOL64, f
