@@ -1964,38 +1964,34 @@ struct M1Sink {
     }
 };
 
-// Adds one contribution (mean of the vertex values when rel is null) with
-// weight w: density, velocity and velocity gradient (G constant per
-// tetrahedron), scalars and their gradients, in the reference's formulas.
-template <typename F, bool VEL, int NS, int K>
-__device__ __forceinline__ void m1_add(const Method1Args &a, const M1Tet &T, const double *rel, double w,
-                                       int64_t cell, const M1Sink<F, K> &sink) {
-    const int slot = sink.find(cell);
+// A tetrahedron's linear fields in the field precision: base values and
+// gradients (densityGrad / velocityGrad / scalarGrad of the reference).
+template <typename F, int NS>
+struct M1Fields {
+    F r0, gr[3];          // density at the base vertex, its gradient
+    F v0[3], G[3][3];     // velocity at the base vertex, G[i][c] = d v_c / d x_i
+    F s0[NS > 0 ? NS : 1], gs[NS > 0 ? NS : 1][3];
+    F mr, mv[3], ms[NS > 0 ? NS : 1];  // means of the four vertex values
+};
+
+template <typename F, bool VEL, int NS>
+__device__ __forceinline__ M1Fields<F, NS> m1_fields(const Method1Args &a, const M1Tet &T) {
+    M1Fields<F, NS> f;
     const int id[4] = {T.v.x, T.v.y, T.v.z, T.v.w};
     F AiF[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) AiF[r][c] = (F)T.Ai[r][c];
-    F rl[3] = {0, 0, 0};
-    if (rel)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) rl[d] = (F)rel[d];
     if (a.fields & DTFE_FIELD_DENSITY) {
         F r[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) r[k] = ldf<F>(a.vf.rho, id[k]);
-        F val;
-        if (!rel) {
-            val = (((r[0] + r[1]) + r[2]) + r[3]) / (F)4;
-        } else {
-            const F dd[3] = {r[1] - r[0], r[2] - r[0], r[3] - r[0]};
-            F g[3];
+        const F dd[3] = {r[1] - r[0], r[2] - r[0], r[3] - r[0]};
 #pragma unroll
-            for (int i = 0; i < 3; ++i) g[i] = AiF[i][0] * dd[0] + AiF[i][1] * dd[1] + AiF[i][2] * dd[2];
-            val = r[0] + (rl[0] * g[0] + rl[1] * g[1] + rl[2] * g[2]);
-        }
-        sink.add(a, cell, slot, 0, (double)val * w);
+        for (int i = 0; i < 3; ++i) f.gr[i] = AiF[i][0] * dd[0] + AiF[i][1] * dd[1] + AiF[i][2] * dd[2];
+        f.r0 = r[0];
+        f.mr = (((r[0] + r[1]) + r[2]) + r[3]) / (F)4;
     }
     if (VEL && (a.fields & (DTFE_FIELD_VELOCITY | kGradUsers))) {
         F vb[4][3];
@@ -2003,50 +1999,132 @@ __device__ __forceinline__ void m1_add(const Method1Args &a, const M1Tet &T, con
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c) vb[k][c] = ldf<F>(a.vf.vel, 3 * (int64_t)id[k] + c);
-        F G[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                G[i][c] = AiF[i][0] * (vb[1][c] - vb[0][c]) + AiF[i][1] * (vb[2][c] - vb[0][c]) +
-                          AiF[i][2] * (vb[3][c] - vb[0][c]);
+                f.G[i][c] = AiF[i][0] * (vb[1][c] - vb[0][c]) + AiF[i][1] * (vb[2][c] - vb[0][c]) +
+                            AiF[i][2] * (vb[3][c] - vb[0][c]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            f.v0[c] = vb[0][c];
+            f.mv[c] = (((vb[0][c] + vb[1][c]) + vb[2][c]) + vb[3][c]) / (F)4;
+        }
+    }
+    if (NS > 0 && (a.fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT))) {
+        const int S = a.vf.S;
+#pragma unroll
+        for (int q = 0; q < (NS > 0 ? NS : 1); ++q) {
+            if (q >= S) break;
+            F sb[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sb[k] = ldf<F>(a.vf.scal, (int64_t)S * id[k] + q);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                f.gs[q][i] = AiF[i][0] * (sb[1] - sb[0]) + AiF[i][1] * (sb[2] - sb[0]) + AiF[i][2] * (sb[3] - sb[0]);
+            f.s0[q] = sb[0];
+            f.ms[q] = (((sb[0] + sb[1]) + sb[2]) + sb[3]) / (F)4;
+        }
+    }
+    return f;
+}
+
+// Points of one tetrahedron that fell in one grid cell: their count and the
+// sum of their positions relative to the base vertex. The fields are linear,
+// so the run's contribution is w (cnt f(base) + grad f . srel).
+struct M1Run {
+    int64_t cell;
+    double cnt;
+    double srel[3];
+};
+
+// Adds a run (mean = the whole tetrahedron in one grid cell, with weight
+// its volume) in the reference's formulas, velocity value rule included.
+template <typename F, bool VEL, int NS, int K>
+__device__ __forceinline__ void m1_flush(const Method1Args &a, const M1Fields<F, NS> &f, const M1Run &r, double w,
+                                         bool mean, const M1Sink<F, K> &sink) {
+    const int slot = sink.find(r.cell);
+    const F cnt = (F)r.cnt;
+    const F sr[3] = {(F)r.srel[0], (F)r.srel[1], (F)r.srel[2]};
+    if (a.fields & DTFE_FIELD_DENSITY) {
+        const F v = mean ? f.mr : cnt * f.r0 + (sr[0] * f.gr[0] + sr[1] * f.gr[1] + sr[2] * f.gr[2]);
+        sink.add(a, r.cell, slot, 0, (double)v * w);
+    }
+    if (VEL && (a.fields & (DTFE_FIELD_VELOCITY | kGradUsers))) {
         if (a.fields & DTFE_FIELD_VELOCITY)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                F val;
-                if (!rel) val = (((vb[0][c] + vb[1][c]) + vb[2][c]) + vb[3][c]) / (F)4;
-                else if (a.value_bug) val = G[2][c] * rl[2] + vb[0][c];
-                else val = (G[0][c] * rl[0] + G[1][c] * rl[1] + G[2][c] * rl[2]) + vb[0][c];
-                sink.add(a, cell, slot, 1 + c, (double)val * w);
+                F v;
+                if (mean) v = f.mv[c];
+                else if (a.value_bug) v = cnt * f.v0[c] + f.G[2][c] * sr[2];
+                else v = cnt * f.v0[c] + (f.G[0][c] * sr[0] + f.G[1][c] * sr[1] + f.G[2][c] * sr[2]);
+                sink.add(a, r.cell, slot, 1 + c, (double)v * w);
             }
         if (a.fields & kGradUsers)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) sink.add(a, cell, slot, 4 + c * 3 + i, (double)G[i][c] * w);
+                for (int i = 0; i < 3; ++i) sink.add(a, r.cell, slot, 4 + c * 3 + i, (double)(cnt * f.G[i][c]) * w);
     }
     if (NS > 0 && (a.fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT))) {
         const int S = a.vf.S;
-        for (int q = 0; q < S; ++q) {
-            F sb[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) sb[k] = ldf<F>(a.vf.scal, (int64_t)S * id[k] + q);
-            F g[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                g[i] = AiF[i][0] * (sb[1] - sb[0]) + AiF[i][1] * (sb[2] - sb[0]) + AiF[i][2] * (sb[3] - sb[0]);
+        for (int q = 0; q < (NS > 0 ? NS : 1); ++q) {
+            if (q >= S) break;
             if (a.fields & DTFE_FIELD_SCALAR) {
-                F val;
-                if (!rel) val = (((sb[0] + sb[1]) + sb[2]) + sb[3]) / (F)4;
-                else if (a.value_bug) val = g[2] * rl[2] + sb[0];
-                else val = (g[0] * rl[0] + g[1] * rl[1] + g[2] * rl[2]) + sb[0];
-                sink.add(a, cell, slot, 13 + q, (double)val * w);
+                F v;
+                if (mean) v = f.ms[q];
+                else if (a.value_bug) v = cnt * f.s0[q] + f.gs[q][2] * sr[2];
+                else v = cnt * f.s0[q] + (f.gs[q][0] * sr[0] + f.gs[q][1] * sr[1] + f.gs[q][2] * sr[2]);
+                sink.add(a, r.cell, slot, 13 + q, (double)v * w);
             }
             if (a.fields & DTFE_FIELD_SCALAR_GRADIENT)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) sink.add(a, cell, slot, 13 + S + 3 * q + i, (double)g[i] * w);
+                for (int i = 0; i < 3; ++i)
+                    sink.add(a, r.cell, slot, 13 + S + 3 * q + i, (double)(cnt * f.gs[q][i]) * w);
         }
     }
+}
+
+constexpr int kM1Runs = 4;  // runs one thread keeps before flushing the oldest
+
+// Points i0, i0 + stride, ... of T, binned into runs per grid cell.
+template <typename F, bool VEL, int NS, int K>
+__device__ __forceinline__ void m1_points(const Method1Args &a, const M1Tet &T, int64_t i0, int64_t stride,
+                                          const M1Sink<F, K> &sink) {
+    if (i0 >= T.np) return;
+    const M1Fields<F, NS> f = m1_fields<F, VEL, NS>(a, T);
+    const double w = T.vol / (double)T.np;
+    M1Run run[kM1Runs];
+    int used = 0;
+    for (int64_t i = i0; i < T.np; i += stride) {
+        double rel[3];
+        const int64_t cell = m1_point(a, T, i, rel);
+        if (cell < 0) continue;
+        bool hit = false;
+#pragma unroll
+        for (int j = 0; j < kM1Runs; ++j)
+            if (j < used && run[j].cell == cell) {
+                run[j].cnt += 1.0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) run[j].srel[d] += rel[d];
+                hit = true;
+            }
+        if (hit) continue;
+        if (used == kM1Runs) {
+            m1_flush<F, VEL, NS, K>(a, f, run[0], w, false, sink);
+#pragma unroll
+            for (int j = 0; j + 1 < kM1Runs; ++j) run[j] = run[j + 1];
+            used = kM1Runs - 1;
+        }
+#pragma unroll
+        for (int j = 0; j < kM1Runs; ++j)
+            if (j == used) run[j] = M1Run{cell, 1.0, {rel[0], rel[1], rel[2]}};
+        ++used;
+    }
+#pragma unroll
+    for (int j = 0; j < kM1Runs; ++j)
+        if (j < used) m1_flush<F, VEL, NS, K>(a, f, run[j], w, false, sink);
 }
 
 // One tile of kBlock consecutive (Morton-ordered) tetrahedra per pass: each
@@ -2061,7 +2139,7 @@ __global__ void __launch_bounds__(kBlock) k_method1(Method1Args a) {
     __shared__ double vals[kSlots * K];
     __shared__ int64_t queue[kBlock];
     __shared__ int nq;
-    const M1Sink<F, K> sink{keys, vals, kSlots};
+    M1Sink<F, K> sink{keys, vals, kSlots};
     for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < a.nc; t0 += (int64_t)gridDim.x * kBlock) {
         for (int i = threadIdx.x; i < kSlots; i += kBlock) keys[i] = -1;
         for (int i = threadIdx.x; i < kSlots * K; i += kBlock) vals[i] = 0.0;
@@ -2071,30 +2149,18 @@ __global__ void __launch_bounds__(kBlock) k_method1(Method1Args a) {
         if (t < a.nc) {
             const M1Tet T = m1_tet(a, t);
             if (T.kind == 1) {
-                m1_add<F, VEL, NS, K>(a, T, nullptr, T.vol, T.cell, sink);
+                const M1Fields<F, NS> f = m1_fields<F, VEL, NS>(a, T);
+                m1_flush<F, VEL, NS, K>(a, f, M1Run{T.cell, 1.0, {0.0, 0.0, 0.0}}, T.vol, true, sink);
             } else if (T.kind == 2) {
-                if (T.np > kM1Coop) {
-                    queue[atomicAdd(&nq, 1)] = t;
-                } else {
-                    const double w = T.vol / (double)T.np;
-                    for (int64_t i = 0; i < T.np; ++i) {
-                        double rel[3];
-                        const int64_t cell = m1_point(a, T, i, rel);
-                        if (cell >= 0) m1_add<F, VEL, NS, K>(a, T, rel, w, cell, sink);
-                    }
-                }
+                if (T.np > kM1Coop) queue[atomicAdd(&nq, 1)] = t;
+                else m1_points<F, VEL, NS, K>(a, T, 0, 1, sink);
             }
         }
         __syncthreads();
         const int n_queued = nq;
         for (int k = 0; k < n_queued; ++k) {
             const M1Tet T = m1_tet(a, queue[k]);
-            const double w = T.vol / (double)T.np;
-            for (int64_t i = threadIdx.x; i < T.np; i += kBlock) {
-                double rel[3];
-                const int64_t cell = m1_point(a, T, i, rel);
-                if (cell >= 0) m1_add<F, VEL, NS, K>(a, T, rel, w, cell, sink);
-            }
+            m1_points<F, VEL, NS, K>(a, T, threadIdx.x, kBlock, sink);
         }
         __syncthreads();
         for (int e = threadIdx.x; e < kSlots * K; e += kBlock) {
