@@ -27,7 +27,7 @@ struct dtfe_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, plan, flags, rows, mass, mom,
-        defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, adj_acc, centre, records, m1_qr, m1_acc,
+        defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, adj_acc, centre, records, m1_qr, m1_bins, m1_list,
         front;  // scratch of the front-end calls (counts, offsets, partial sums, flags)
     int last_deferred = 0;
     int64_t m1_points = 0;  // quasi-random points held in m1_qr (averaging method 1)

@@ -1230,6 +1230,11 @@ __device__ __forceinline__ T dpp_move_t(T v) {
     }
 }
 
+// Lanes of the wave below this one whose bit is set in b.
+__device__ __forceinline__ int lanes_below(uint64_t b) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
 template <int LPR, typename T>
 __device__ __forceinline__ T lane_sum(T v) {
     static_assert(LPR == 2 || LPR == 4 || LPR == 8 || LPR == 16, "lane groups of 2, 4, 8 or 16");
@@ -1834,11 +1839,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_tets(const double *__restrict__ 
 // Every tetrahedron whose bounding box meets the region either lies inside
 // one grid cell and adds mean(vertex values) * volume to it, or takes n
 // quasi-random points (n from its volume in grid cells) and adds
-// value(point) * volume / n to the grid cell of each point. Per-cell sums
-// are packed K = 13 + 4 S values per grid cell (density, velocity, velocity
-// gradient, scalars, scalar gradients) and merged per workgroup in an LDS
-// hash table before the HBM atomics; k_method1_finish divides by the grid
-// cell volume and writes the rows with the derived velocity fields.
+// value(point) * volume / n to the grid cell of each point. Tetrahedra are
+// binned to bricks of grid cells (k_m1_bin); one workgroup per brick sums
+// the K = 13 + 4 S values per cell (density, velocity, velocity gradient,
+// scalars, scalar gradients) in LDS and writes its rows divided by the grid
+// cell volume, with the derived velocity fields (k_m1_brick).
 struct Method1Args {
     const double *pos;
     const int4 *cells;
@@ -1846,17 +1851,24 @@ struct Method1Args {
     VertexFieldsDev vf;
     double box[6];
     double dx[3];
+    double inv_dx[3];
     int n[3];
     double grid_volume;
     int64_t NN, minNN, maxNN;
     const double *qr;  // [maxNN][3] quasi-random points of the unit simplex
     unsigned fields;
     bool value_bug;
-    void *acc;         // [ncell][K]
-    int K;             // 13 + 4 S
 };
 
-constexpr int kM1Coop = 64;  // tetrahedra with more points go to the whole workgroup
+// floor(x / d) as an int, the reference's int(floor(x / dx)), from a
+// reciprocal multiply: the two quotients differ by a few ulp, so only a
+// product within 1e-9 of an integer needs the division.
+__device__ __forceinline__ int floor_div(double x, double d, double inv_d) {
+    const double y = x * inv_d;
+    const double r = rint(y);
+    if (fabs(y - r) <= 1e-9 * fmax(1.0, fabs(y))) return (int)floor(x / d);
+    return (int)floor(y);
+}
 
 // What one tetrahedron does (checkCellPosition, :87-122, and the point count, :361-364).
 struct M1Tet {
@@ -1896,7 +1908,7 @@ __device__ __forceinline__ M1Tet m1_tet(const Method1Args &a, int64_t t) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             T.base[i] = xs[0][i] - a.box[2 * i];
-            g[i] = (int64_t)(int)floor(T.base[i] / a.dx[i]);
+            g[i] = floor_div(T.base[i], a.dx[i], a.inv_dx[i]);
             const double lo = (double)g[i] * a.dx[i] + a.box[2 * i];
             const double hi = lo + a.dx[i];
 #pragma unroll
@@ -1925,44 +1937,20 @@ __device__ __forceinline__ M1Tet m1_tet(const Method1Args &a, int64_t t) {
     return T;
 }
 
-// Grid cell of quasi-random point i of T (-1 outside the grid) and its
-// position relative to the base vertex.
-__device__ __forceinline__ int64_t m1_point(const Method1Args &a, const M1Tet &T, int64_t i, double rel[3]) {
+// Grid cell g[3] of quasi-random point i of T (false outside the grid) and
+// the point's position relative to the base vertex.
+__device__ __forceinline__ bool m1_point(const Method1Args &a, const M1Tet &T, int64_t i, double rel[3], int g[3]) {
 #pragma clang fp contract(off)
     const double q0 = a.qr[3 * i], q1 = a.qr[3 * i + 1], q2 = a.qr[3 * i + 2];
-    int64_t g[3];
+    bool in = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         rel[d] = q0 * T.vM[0][d] + q1 * T.vM[1][d] + q2 * T.vM[2][d];
-        g[d] = (int64_t)(int)floor((T.base[d] + rel[d]) / a.dx[d]);
-        if (g[d] < 0 || g[d] >= a.n[d]) return -1;
+        g[d] = floor_div(T.base[d] + rel[d], a.dx[d], a.inv_dx[d]);
+        in = in && g[d] >= 0 && g[d] < a.n[d];
     }
-    return (g[0] * a.n[1] + g[1]) * a.n[2] + g[2];
+    return in;
 }
-
-// The per-cell sums of one workgroup: LDS slots keyed by grid cell, K FP64
-// values each; a cell that finds no slot goes straight to HBM.
-template <typename F, int K>
-struct M1Sink {
-    int64_t *keys;   // [S] -1 = free
-    double *vals;    // [S][K]
-    int slots;
-    __device__ int find(int64_t cell) const {
-        unsigned h = (unsigned)(((uint64_t)cell * 0x9E3779B97F4A7C15ull) >> 40) & (unsigned)(slots - 1);
-        for (int probe = 0; probe < 32; ++probe) {
-            const unsigned long long k =
-                atomicCAS((unsigned long long *)&keys[h], (unsigned long long)-1, (unsigned long long)cell);
-            if ((long long)k == -1 || (int64_t)k == cell) return (int)h;
-            h = (h + 1) & (unsigned)(slots - 1);
-        }
-        return -1;
-    }
-    __device__ void add(const Method1Args &a, int64_t cell, int slot, int c, double v) const {
-        if (v == 0.0) return;
-        if (slot >= 0) atomicAdd(&vals[slot * K + c], v);
-        else atomic_add_f<F>(a.acc, cell * a.K + c, (F)v);
-    }
-};
 
 // A tetrahedron's linear fields in the field precision: base values and
 // gradients (densityGrad / velocityGrad / scalarGrad of the reference).
@@ -2029,42 +2017,35 @@ __device__ __forceinline__ M1Fields<F, NS> m1_fields(const Method1Args &a, const
     return f;
 }
 
-// Points of one tetrahedron that fell in one grid cell: their count and the
-// sum of their positions relative to the base vertex. The fields are linear,
-// so the run's contribution is w (cnt f(base) + grad f . srel).
-struct M1Run {
-    int64_t cell;
-    double cnt;
-    double srel[3];
-};
-
-// Adds a run (mean = the whole tetrahedron in one grid cell, with weight
-// its volume) in the reference's formulas, velocity value rule included.
+// Adds one point's contributions (the tetrahedron's means when rel is null,
+// with weight its volume) to the LDS sums r[K] of its grid cell, in the
+// reference's formulas, velocity value rule included.
 template <typename F, bool VEL, int NS, int K>
-__device__ __forceinline__ void m1_flush(const Method1Args &a, const M1Fields<F, NS> &f, const M1Run &r, double w,
-                                         bool mean, const M1Sink<F, K> &sink) {
-    const int slot = sink.find(r.cell);
-    const F cnt = (F)r.cnt;
-    const F sr[3] = {(F)r.srel[0], (F)r.srel[1], (F)r.srel[2]};
+__device__ __forceinline__ void m1_add_local(const Method1Args &a, const M1Fields<F, NS> &f, const double *rel,
+                                             double w, double *r) {
+    F rl[3] = {0, 0, 0};
+    if (rel)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) rl[d] = (F)rel[d];
     if (a.fields & DTFE_FIELD_DENSITY) {
-        const F v = mean ? f.mr : cnt * f.r0 + (sr[0] * f.gr[0] + sr[1] * f.gr[1] + sr[2] * f.gr[2]);
-        sink.add(a, r.cell, slot, 0, (double)v * w);
+        const F v = rel ? f.r0 + (rl[0] * f.gr[0] + rl[1] * f.gr[1] + rl[2] * f.gr[2]) : f.mr;
+        atomicAdd(r, (double)v * w);
     }
     if (VEL && (a.fields & (DTFE_FIELD_VELOCITY | kGradUsers))) {
         if (a.fields & DTFE_FIELD_VELOCITY)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 F v;
-                if (mean) v = f.mv[c];
-                else if (a.value_bug) v = cnt * f.v0[c] + f.G[2][c] * sr[2];
-                else v = cnt * f.v0[c] + (f.G[0][c] * sr[0] + f.G[1][c] * sr[1] + f.G[2][c] * sr[2]);
-                sink.add(a, r.cell, slot, 1 + c, (double)v * w);
+                if (!rel) v = f.mv[c];
+                else if (a.value_bug) v = f.G[2][c] * rl[2] + f.v0[c];
+                else v = (f.G[0][c] * rl[0] + f.G[1][c] * rl[1] + f.G[2][c] * rl[2]) + f.v0[c];
+                atomicAdd(r + 1 + c, (double)v * w);
             }
         if (a.fields & kGradUsers)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) sink.add(a, r.cell, slot, 4 + c * 3 + i, (double)(cnt * f.G[i][c]) * w);
+                for (int i = 0; i < 3; ++i) atomicAdd(r + 4 + c * 3 + i, (double)f.G[i][c] * w);
     }
     if (NS > 0 && (a.fields & (DTFE_FIELD_SCALAR | DTFE_FIELD_SCALAR_GRADIENT))) {
         const int S = a.vf.S;
@@ -2073,126 +2054,199 @@ __device__ __forceinline__ void m1_flush(const Method1Args &a, const M1Fields<F,
             if (q >= S) break;
             if (a.fields & DTFE_FIELD_SCALAR) {
                 F v;
-                if (mean) v = f.ms[q];
-                else if (a.value_bug) v = cnt * f.s0[q] + f.gs[q][2] * sr[2];
-                else v = cnt * f.s0[q] + (f.gs[q][0] * sr[0] + f.gs[q][1] * sr[1] + f.gs[q][2] * sr[2]);
-                sink.add(a, r.cell, slot, 13 + q, (double)v * w);
+                if (!rel) v = f.ms[q];
+                else if (a.value_bug) v = f.gs[q][2] * rl[2] + f.s0[q];
+                else v = (f.gs[q][0] * rl[0] + f.gs[q][1] * rl[1] + f.gs[q][2] * rl[2]) + f.s0[q];
+                atomicAdd(r + 13 + q, (double)v * w);
             }
             if (a.fields & DTFE_FIELD_SCALAR_GRADIENT)
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    sink.add(a, r.cell, slot, 13 + S + 3 * q + i, (double)(cnt * f.gs[q][i]) * w);
+                for (int i = 0; i < 3; ++i) atomicAdd(r + 13 + S + 3 * q + i, (double)f.gs[q][i] * w);
         }
     }
 }
 
-constexpr int kM1Runs = 4;  // runs one thread keeps before flushing the oldest
+// Grid bricks of BX x BY x BZ cells: a workgroup owns one brick, sums the
+// contributions of every tetrahedron binned to it in LDS (no hashing, no
+// atomics in HBM) and writes its rows.
+struct M1Bricks {
+    int b[3];    // cells per brick
+    int nb[3];   // bricks per axis
+    int64_t count() const { return (int64_t)nb[0] * nb[1] * nb[2]; }
+};
 
-// Points i0, i0 + stride, ... of T, binned into runs per grid cell.
-template <typename F, bool VEL, int NS, int K>
-__device__ __forceinline__ void m1_points(const Method1Args &a, const M1Tet &T, int64_t i0, int64_t stride,
-                                          const M1Sink<F, K> &sink) {
-    if (i0 >= T.np) return;
-    const M1Fields<F, NS> f = m1_fields<F, VEL, NS>(a, T);
-    const double w = T.vol / (double)T.np;
-    M1Run run[kM1Runs];
-    int used = 0;
-    for (int64_t i = i0; i < T.np; i += stride) {
-        double rel[3];
-        const int64_t cell = m1_point(a, T, i, rel);
-        if (cell < 0) continue;
-        bool hit = false;
+// Bricks holding the grid cells a tetrahedron can add to: the cell of a
+// one-cell tetrahedron, else every cell its points can reach, from the
+// bounds of base + q vertexMatrix widened by far more than their rounding.
+__device__ __forceinline__ bool m1_brick_range(const Method1Args &a, const M1Tet &T, const M1Bricks &B, int lo[3],
+                                               int hi[3]) {
+    if (T.kind == 0) return false;
+    if (T.kind == 1) {
+        int64_t c = T.cell;
+        const int g2 = (int)(c % a.n[2]);
+        c /= a.n[2];
+        const int g1 = (int)(c % a.n[1]);
+        const int g0 = (int)(c / a.n[1]);
+        const int g[3] = {g0, g1, g2};
 #pragma unroll
-        for (int j = 0; j < kM1Runs; ++j)
-            if (j < used && run[j].cell == cell) {
-                run[j].cnt += 1.0;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) run[j].srel[d] += rel[d];
-                hit = true;
-            }
-        if (hit) continue;
-        if (used == kM1Runs) {
-            m1_flush<F, VEL, NS, K>(a, f, run[0], w, false, sink);
-#pragma unroll
-            for (int j = 0; j + 1 < kM1Runs; ++j) run[j] = run[j + 1];
-            used = kM1Runs - 1;
-        }
-#pragma unroll
-        for (int j = 0; j < kM1Runs; ++j)
-            if (j == used) run[j] = M1Run{cell, 1.0, {rel[0], rel[1], rel[2]}};
-        ++used;
+        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = g[d] / B.b[d];
+        return true;
     }
 #pragma unroll
-    for (int j = 0; j < kM1Runs; ++j)
-        if (j < used) m1_flush<F, VEL, NS, K>(a, f, run[j], w, false, sink);
+    for (int d = 0; d < 3; ++d) {
+        const double m0 = fmin(0.0, fmin(T.vM[0][d], fmin(T.vM[1][d], T.vM[2][d])));
+        const double m1 = fmax(0.0, fmax(T.vM[0][d], fmax(T.vM[1][d], T.vM[2][d])));
+        const double eps =
+            1e-9 * (fabs(T.base[d]) + fabs(T.vM[0][d]) + fabs(T.vM[1][d]) + fabs(T.vM[2][d]) + a.dx[d]);
+        const double clo = floor((T.base[d] + m0 - eps) / a.dx[d]);
+        const double chi = floor((T.base[d] + m1 + eps) / a.dx[d]);
+        if (chi < 0.0 || clo > (double)(a.n[d] - 1)) return false;
+        const int c0 = (int)fmax(clo, 0.0), c1 = (int)fmin(chi, (double)(a.n[d] - 1));
+        lo[d] = c0 / B.b[d];
+        hi[d] = c1 / B.b[d];
+    }
+    return true;
 }
 
-// One tile of kBlock consecutive (Morton-ordered) tetrahedra per pass: each
-// thread adds its own tetrahedron; those with more than kM1Coop points are
-// queued and their points shared by the whole workgroup; then the merged
-// cells go to HBM, one value per lane.
-template <typename F, bool VEL, int NS>
-__global__ void __launch_bounds__(kBlock) k_method1(Method1Args a) {
-    constexpr int K = 13 + 4 * NS;
-    constexpr int kSlots = NS > 0 ? 256 : 512;
-    __shared__ int64_t keys[kSlots];
-    __shared__ double vals[kSlots * K];
-    __shared__ int64_t queue[kBlock];
-    __shared__ int nq;
-    M1Sink<F, K> sink{keys, vals, kSlots};
-    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < a.nc; t0 += (int64_t)gridDim.x * kBlock) {
-        for (int i = threadIdx.x; i < kSlots; i += kBlock) keys[i] = -1;
-        for (int i = threadIdx.x; i < kSlots * K; i += kBlock) vals[i] = 0.0;
-        if (threadIdx.x == 0) nq = 0;
-        __syncthreads();
-        const int64_t t = t0 + threadIdx.x;
+// Pass 0 counts, pass 1 lists, the tetrahedra of every brick (list
+// positions from cursor[], which starts at each brick's offset). Lanes of a
+// wave that target the same brick (Morton-ordered tetrahedra mostly do)
+// share one atomic.
+template <int PASS>
+__global__ void __launch_bounds__(kBlock) k_m1_bin(Method1Args a, M1Bricks B, int *__restrict__ count,
+                                                   int64_t *__restrict__ cursor, int32_t *__restrict__ list) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n_iter = (a.nc + stride - 1) / stride;  // uniform trip count: every lane takes part
+    for (int64_t it = 0; it < n_iter; ++it) {
+        const int64_t t = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
         if (t < a.nc) {
             const M1Tet T = m1_tet(a, t);
-            if (T.kind == 1) {
-                const M1Fields<F, NS> f = m1_fields<F, VEL, NS>(a, T);
-                m1_flush<F, VEL, NS, K>(a, f, M1Run{T.cell, 1.0, {0.0, 0.0, 0.0}}, T.vol, true, sink);
-            } else if (T.kind == 2) {
-                if (T.np > kM1Coop) queue[atomicAdd(&nq, 1)] = t;
-                else m1_points<F, VEL, NS, K>(a, T, 0, 1, sink);
+            if (!m1_brick_range(a, T, B, lo, hi)) hi[0] = -1;
+        }
+        const int span1 = hi[1] - lo[1] + 1, span2 = hi[2] - lo[2] + 1;
+        const int mine = hi[0] < lo[0] ? 0 : (hi[0] - lo[0] + 1) * span1 * span2;
+        int most = mine;
+        for (int d = 32; d > 0; d >>= 1) most = max(most, __shfl_xor(most, d, 64));
+        for (int k = 0; k < most; ++k) {
+            int64_t b = -1;
+            if (k < mine) {
+                const int i = lo[0] + k / (span1 * span2), j = lo[1] + (k / span2) % span1, l = lo[2] + k % span2;
+                b = ((int64_t)i * B.nb[1] + j) * B.nb[2] + l;
+            }
+            bool pending = b >= 0;
+            while (__ballot(pending)) {
+                const uint64_t act = __ballot(pending);
+                const int leader = __builtin_ctzll(act);
+                const int64_t lb = __shfl(b, leader, 64);
+                const uint64_t same = __ballot(pending && b == lb);
+                if (PASS == 0) {
+                    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&count[lb], __popcll(same));
+                } else {
+                    int64_t base = 0;
+                    if ((int)(threadIdx.x & 63) == leader)
+                        base = (int64_t)atomicAdd((unsigned long long *)&cursor[lb], (unsigned long long)__popcll(same));
+                    base = __shfl(base, leader, 64);
+                    if (pending && b == lb) list[base + lanes_below(same)] = (int32_t)t;
+                }
+                if (pending && b == lb) pending = false;
             }
         }
-        __syncthreads();
-        const int n_queued = nq;
-        for (int k = 0; k < n_queued; ++k) {
-            const M1Tet T = m1_tet(a, queue[k]);
-            m1_points<F, VEL, NS, K>(a, T, threadIdx.x, kBlock, sink);
-        }
-        __syncthreads();
-        for (int e = threadIdx.x; e < kSlots * K; e += kBlock) {
-            const int sl = e / K, c = e - sl * K;
-            const int64_t cell = keys[sl];
-            const double v = vals[e];
-            if (cell >= 0 && c < a.K && v != 0.0) atomic_add_f<F>(a.acc, cell * a.K + c, (F)v);
-        }
-        __syncthreads();
     }
 }
 
-// Per-cell sums / grid cell volume -> output rows (with divergence, shear
-// and vorticity of the averaged gradient).
-template <typename F, bool VEL, int NS>
-__global__ void __launch_bounds__(kBlock) k_method1_finish(const F *__restrict__ acc, int64_t ncell, int K, int S,
-                                                           unsigned fields, F scale, OutPtrs out) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += stride) {
-        const F *r = acc + c * K;
+// offsets[i] = sum of count[0..i) for i <= n (one workgroup of 1024).
+__global__ void __launch_bounds__(1024) k_m1_scan(const int *__restrict__ count, int64_t n,
+                                                  int64_t *__restrict__ offsets) {
+    __shared__ int64_t wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t seg = ((n + 15) / 16 + 63) / 64 * 64;
+    const int64_t lo = min(n, (int64_t)w * seg), hi = min(n, lo + seg);
+    int64_t s = 0;
+    for (int64_t i = lo + lane; i < hi; i += 64) s += count[i];
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) wsum[w] = s;
+    __syncthreads();
+    int64_t carry = 0, total = 0;
+    for (int v = 0; v < 16; ++v) {
+        if (v < w) carry += wsum[v];
+        total += wsum[v];
+    }
+    for (int64_t c0 = lo; c0 < hi; c0 += 64) {
+        const int64_t i = c0 + lane;
+        const int64_t x = i < hi ? count[i] : 0;
+        int64_t inc = x;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += y;
+        }
+        if (i < hi) offsets[i] = carry + inc - x;
+        carry += __shfl(inc, 63, 64);
+    }
+    if (threadIdx.x == 0) offsets[n] = total;
+}
+
+// One workgroup per brick: every tetrahedron of its list adds its
+// contributions inside the brick to LDS sums (FP64), which are then divided
+// by the grid cell volume and written as rows with the derived fields.
+template <typename F, bool VEL, int NS, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) k_m1_brick(Method1Args a, M1Bricks B, const int64_t *__restrict__ offsets,
+                                                     const int32_t *__restrict__ list, F scale, OutPtrs out) {
+    constexpr int K = 13 + 4 * NS;
+    constexpr int kCells = BX * BY * BZ;
+    __shared__ double acc[kCells * K];
+    const int64_t b = blockIdx.x;
+    const int bk = (int)(b % B.nb[2]), bj = (int)((b / B.nb[2]) % B.nb[1]), bi = (int)(b / ((int64_t)B.nb[1] * B.nb[2]));
+    const int c0[3] = {bi * BX, bj * BY, bk * BZ};
+    for (int i = threadIdx.x; i < kCells * K; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    // the local slot of grid cell `cell`, -1 outside this brick
+    auto local = [&](int64_t cell) -> int {
+        const int g2 = (int)(cell % a.n[2]);
+        const int64_t r = cell / a.n[2];
+        const int g1 = (int)(r % a.n[1]), g0 = (int)(r / a.n[1]);
+        const int l0 = g0 - c0[0], l1 = g1 - c0[1], l2 = g2 - c0[2];
+        if (l0 < 0 || l0 >= BX || l1 < 0 || l1 >= BY || l2 < 0 || l2 >= BZ) return -1;
+        return (l0 * BY + l1) * BZ + l2;
+    };
+    const int64_t e0 = offsets[b], e1 = offsets[b + 1];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const M1Tet T = m1_tet(a, list[e]);
+        const M1Fields<F, NS> f = m1_fields<F, VEL, NS>(a, T);
+        if (T.kind == 1) {
+            const int l = local(T.cell);
+            if (l >= 0) m1_add_local<F, VEL, NS, K>(a, f, nullptr, T.vol, acc + l * K);
+            continue;
+        }
+        const double w = T.vol / (double)T.np;
+        for (int64_t i = 0; i < T.np; ++i) {
+            double rel[3];
+            int g[3];
+            if (!m1_point(a, T, i, rel, g)) continue;
+            const int l0 = g[0] - c0[0], l1 = g[1] - c0[1], l2 = g[2] - c0[2];
+            if (l0 < 0 || l0 >= BX || l1 < 0 || l1 >= BY || l2 < 0 || l2 >= BZ) continue;
+            m1_add_local<F, VEL, NS, K>(a, f, rel, w, acc + ((l0 * BY + l1) * BZ + l2) * K);
+        }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < kCells; l += blockDim.x) {
+        const int l2 = l % BZ, l1 = (l / BZ) % BY, l0 = l / (BY * BZ);
+        const int g0 = c0[0] + l0, g1 = c0[1] + l1, g2 = c0[2] + l2;
+        if (g0 >= a.n[0] || g1 >= a.n[1] || g2 >= a.n[2]) continue;
+        const double *r = acc + l * K;
         RowAcc<F, NS> ra;
         ra.clear();
-        ra.d = r[0];
+        ra.d = (F)r[0];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) ra.v[i] = r[1 + i];
+        for (int i = 0; i < 3; ++i) ra.v[i] = (F)r[1 + i];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) ra.vg[i] = r[4 + i];
+        for (int i = 0; i < 9; ++i) ra.vg[i] = (F)r[4 + i];
         if (NS > 0) {
-            for (int q = 0; q < S; ++q) ra.s[q] = r[13 + q];
-            for (int q = 0; q < 3 * S; ++q) ra.sg[q] = r[13 + S + q];
+            for (int q = 0; q < a.vf.S; ++q) ra.s[q] = (F)r[13 + q];
+            for (int q = 0; q < 3 * a.vf.S; ++q) ra.sg[q] = (F)r[13 + a.vf.S + q];
         }
-        store_row<F, VEL, NS>(out, c, ra, scale, fields, S);
+        const int64_t row = ((int64_t)g0 * a.n[1] + g1) * a.n[2] + g2;
+        store_row<F, VEL, NS>(out, row, ra, scale, a.fields, a.vf.S);
     }
 }
 
@@ -2824,14 +2878,25 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     return DTFE_OK;
 }
 
-// Averaging method 1 on the regular grid of `a` (rows = grid cells).
+template <typename F, bool VEL, int NS>
+void launch_m1_bricks(const Method1Args &ma, const M1Bricks &B, const int64_t *offsets, const int32_t *list,
+                      double scale, const OutPtrs &out, hipStream_t s) {
+    const dim3 g((unsigned)B.count());
+    if (NS > 0)
+        hipLaunchKernelGGL((k_m1_brick<F, VEL, NS, 8, 8, 4>), g, dim3(kBlock), 0, s, ma, B, offsets, list, (F)scale,
+                           out);
+    else
+        hipLaunchKernelGGL((k_m1_brick<F, VEL, NS, 8, 8, 8>), g, dim3(kBlock), 0, s, ma, B, offsets, list, (F)scale,
+                           out);
+}
+
+// Averaging method 1 on the regular grid of `a` (rows = grid cells): bin the
+// tetrahedra to bricks (count, scan, list; one wait for the list size), then
+// one workgroup per brick.
 int launch_method1(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *vf, unsigned fields, int S,
                    const double box[6], int64_t NN, unsigned flags, const SampleArgs &a) {
     int rc;
     const bool f64 = vf->dtype == DTFE_F64;
-    const size_t esz = f64 ? 8 : 4;
-    const int K = 13 + 4 * S;
-    const int64_t ncell = a.n_rows;
     const int64_t maxNN = 100 * NN;
     if (ctx->m1_points != maxNN) {
         const std::vector<double> q = sobol_simplex(maxNN);
@@ -2839,13 +2904,6 @@ int launch_method1(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *
         HIP_TRY(hipMemcpy(ctx->m1_qr.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
         ctx->m1_points = maxNN;
     }
-    if ((rc = ensure(ctx, ctx->m1_acc, (size_t)ncell * K * esz))) return rc;
-    if (f64)
-        hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(ncell * K)), dim3(kBlock), 0, ctx->stream,
-                           (double *)ctx->m1_acc.p, ncell * K, 0.0);
-    else
-        hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(ncell * K)), dim3(kBlock), 0, ctx->stream,
-                           (float *)ctx->m1_acc.p, ncell * K, 0.0f);
     Method1Args ma{};
     ma.pos = m->pos;
     ma.cells = (const int4 *)m->cells;
@@ -2854,6 +2912,7 @@ int launch_method1(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *
     for (int i = 0; i < 6; ++i) ma.box[i] = box[i];
     for (int i = 0; i < 3; ++i) {
         ma.dx[i] = a.dx[i];
+        ma.inv_dx[i] = 1.0 / a.dx[i];
         ma.n[i] = (int)a.n[i];
     }
     ma.grid_volume = a.dx[0] * a.dx[1] * a.dx[2];
@@ -2863,43 +2922,55 @@ int launch_method1(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *
     ma.qr = (const double *)ctx->m1_qr.p;
     ma.fields = fields;
     ma.value_bug = (flags & DTFE_COMPAT_VALUE_BUG) != 0;
-    ma.acc = ctx->m1_acc.p;
-    ma.K = K;
-    const bool vel = (fields & (DTFE_FIELD_VELOCITY | kGradUsers)) != 0;
-    const dim3 g(grid_for(m->n_cells));
+    M1Bricks B;
+    B.b[0] = 8;
+    B.b[1] = 8;
+    B.b[2] = S > 0 ? 4 : 8;
+    for (int d = 0; d < 3; ++d) B.nb[d] = (int)((a.n[d] + B.b[d] - 1) / B.b[d]);
+    const int64_t nb = B.count();
+    // bin buffer: counts [nb] int, offsets and cursors [nb + 1] int64 each
+    const size_t cnt_bytes = ((size_t)nb * sizeof(int) + 255) / 256 * 256;
+    const size_t off_bytes = ((size_t)(nb + 1) * sizeof(int64_t) + 255) / 256 * 256;
+    if ((rc = ensure(ctx, ctx->m1_bins, cnt_bytes + 2 * off_bytes))) return rc;
+    int *count = (int *)ctx->m1_bins.p;
+    int64_t *offsets = (int64_t *)((char *)ctx->m1_bins.p + cnt_bytes);
+    int64_t *cursor = (int64_t *)((char *)ctx->m1_bins.p + cnt_bytes + off_bytes);
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, count, nb, 0);
+    const dim3 gt(grid_for(m->n_cells));
+    hipLaunchKernelGGL(k_m1_bin<0>, gt, dim3(kBlock), 0, ctx->stream, ma, B, count, (int64_t *)nullptr,
+                       (int32_t *)nullptr);
+    hipLaunchKernelGGL(k_m1_scan, dim3(1), dim3(1024), 0, ctx->stream, (const int *)count, nb, offsets);
+    HIP_TRY(hipGetLastError());
+    int64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, offsets + nb, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = ensure(ctx, ctx->m1_list, (size_t)std::max<int64_t>(total, 1) * sizeof(int32_t)))) return rc;
+    HIP_TRY(hipMemcpyAsync(cursor, offsets, (size_t)(nb + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    hipLaunchKernelGGL(k_m1_bin<1>, gt, dim3(kBlock), 0, ctx->stream, ma, B, (int *)nullptr, cursor,
+                       (int32_t *)ctx->m1_list.p);
+    const double scale = 1.0 / ma.grid_volume;
+    const int32_t *list = (const int32_t *)ctx->m1_list.p;
+    const bool vel = (fields & (DTFE_FIELD_VELOCITY | kGradUsers)) != 0;
     if (f64) {
         if (vel) {
-            if (S) hipLaunchKernelGGL((k_method1<double, true, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
-            else hipLaunchKernelGGL((k_method1<double, true, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+            if (S) launch_m1_bricks<double, true, kMaxScalars>(ma, B, offsets, list, scale, a.out, ctx->stream);
+            else launch_m1_bricks<double, true, 0>(ma, B, offsets, list, scale, a.out, ctx->stream);
         } else {
-            if (S) hipLaunchKernelGGL((k_method1<double, false, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
-            else hipLaunchKernelGGL((k_method1<double, false, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+            if (S) launch_m1_bricks<double, false, kMaxScalars>(ma, B, offsets, list, scale, a.out, ctx->stream);
+            else launch_m1_bricks<double, false, 0>(ma, B, offsets, list, scale, a.out, ctx->stream);
         }
     } else {
         if (vel) {
-            if (S) hipLaunchKernelGGL((k_method1<float, true, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
-            else hipLaunchKernelGGL((k_method1<float, true, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+            if (S) launch_m1_bricks<float, true, kMaxScalars>(ma, B, offsets, list, scale, a.out, ctx->stream);
+            else launch_m1_bricks<float, true, 0>(ma, B, offsets, list, scale, a.out, ctx->stream);
         } else {
-            if (S) hipLaunchKernelGGL((k_method1<float, false, kMaxScalars>), g, dim3(kBlock), 0, ctx->stream, ma);
-            else hipLaunchKernelGGL((k_method1<float, false, 0>), g, dim3(kBlock), 0, ctx->stream, ma);
+            if (S) launch_m1_bricks<float, false, kMaxScalars>(ma, B, offsets, list, scale, a.out, ctx->stream);
+            else launch_m1_bricks<float, false, 0>(ma, B, offsets, list, scale, a.out, ctx->stream);
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-    const dim3 gf(grid_for(ncell));
-    if (f64) {
-        const double scale = 1.0 / ma.grid_volume;
-        if (S) hipLaunchKernelGGL((k_method1_finish<double, true, kMaxScalars>), gf, dim3(kBlock), 0, ctx->stream,
-                                  (const double *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
-        else hipLaunchKernelGGL((k_method1_finish<double, true, 0>), gf, dim3(kBlock), 0, ctx->stream,
-                                (const double *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
-    } else {
-        const float scale = (float)(1.0 / ma.grid_volume);
-        if (S) hipLaunchKernelGGL((k_method1_finish<float, true, kMaxScalars>), gf, dim3(kBlock), 0, ctx->stream,
-                                  (const float *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
-        else hipLaunchKernelGGL((k_method1_finish<float, true, 0>), gf, dim3(kBlock), 0, ctx->stream,
-                                (const float *)ctx->m1_acc.p, ncell, K, S, fields, scale, a.out);
-    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
@@ -2947,7 +3018,7 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
                       &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->m1_qr,
-                      &ctx->m1_acc, &ctx->front};
+                      &ctx->m1_bins, &ctx->m1_list, &ctx->front};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (int i = 0; i < 9; ++i)
