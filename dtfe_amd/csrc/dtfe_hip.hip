@@ -2108,21 +2108,34 @@ __device__ __forceinline__ bool m1_brick_range(const Method1Args &a, const M1Tet
     return true;
 }
 
-// Pass 0 counts, pass 1 lists, the tetrahedra of every brick (list
+// Work items of a brick: (tetrahedron, chunk of up to kM1Chunk of its
+// points), packed as t * 2^16 + chunk; a one-cell tetrahedron is one item.
+// Chunks keep the per-lane work of k_m1_brick within a small bound however
+// many points a tetrahedron has.
+constexpr int kM1Chunk = 16;
+
+__device__ __forceinline__ int m1_chunks(const M1Tet &T) {
+    return T.kind == 1 ? 1 : (int)((T.np + kM1Chunk - 1) / kM1Chunk);
+}
+
+// Pass 0 counts, pass 1 lists, the work items of every brick (list
 // positions from cursor[], which starts at each brick's offset). Lanes of a
 // wave that target the same brick (Morton-ordered tetrahedra mostly do)
 // share one atomic.
 template <int PASS>
 __global__ void __launch_bounds__(kBlock) k_m1_bin(Method1Args a, M1Bricks B, int *__restrict__ count,
-                                                   int64_t *__restrict__ cursor, int32_t *__restrict__ list) {
+                                                   int64_t *__restrict__ cursor, int64_t *__restrict__ list) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n_iter = (a.nc + stride - 1) / stride;  // uniform trip count: every lane takes part
+    const int lane = threadIdx.x & 63;
     for (int64_t it = 0; it < n_iter; ++it) {
         const int64_t t = it * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+        int nch = 0;
         if (t < a.nc) {
             const M1Tet T = m1_tet(a, t);
             if (!m1_brick_range(a, T, B, lo, hi)) hi[0] = -1;
+            else nch = m1_chunks(T);
         }
         const int span1 = hi[1] - lo[1] + 1, span2 = hi[2] - lo[2] + 1;
         const int mine = hi[0] < lo[0] ? 0 : (hi[0] - lo[0] + 1) * span1 * span2;
@@ -2139,17 +2152,25 @@ __global__ void __launch_bounds__(kBlock) k_m1_bin(Method1Args a, M1Bricks B, in
                 const uint64_t act = __ballot(pending);
                 const int leader = __builtin_ctzll(act);
                 const int64_t lb = __shfl(b, leader, 64);
-                const uint64_t same = __ballot(pending && b == lb);
+                const bool mate = pending && b == lb;
+                // inclusive prefix of the chunk counts over the lanes of this brick
+                int incl = mate ? nch : 0;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int y = __shfl_up(incl, d, 64);
+                    if (lane >= d) incl += y;
+                }
+                const int total = __shfl(incl, 63, 64);
                 if (PASS == 0) {
-                    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&count[lb], __popcll(same));
+                    if (lane == leader) atomicAdd(&count[lb], total);
                 } else {
                     int64_t base = 0;
-                    if ((int)(threadIdx.x & 63) == leader)
-                        base = (int64_t)atomicAdd((unsigned long long *)&cursor[lb], (unsigned long long)__popcll(same));
+                    if (lane == leader)
+                        base = (int64_t)atomicAdd((unsigned long long *)&cursor[lb], (unsigned long long)total);
                     base = __shfl(base, leader, 64);
-                    if (pending && b == lb) list[base + lanes_below(same)] = (int32_t)t;
+                    if (mate)
+                        for (int c = 0; c < nch; ++c) list[base + incl - nch + c] = (t << 16) | c;
                 }
-                if (pending && b == lb) pending = false;
+                if (mate) pending = false;
             }
         }
     }
@@ -2191,7 +2212,7 @@ __global__ void __launch_bounds__(1024) k_m1_scan(const int *__restrict__ count,
 // by the grid cell volume and written as rows with the derived fields.
 template <typename F, bool VEL, int NS, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(kBlock) k_m1_brick(Method1Args a, M1Bricks B, const int64_t *__restrict__ offsets,
-                                                     const int32_t *__restrict__ list, F scale, OutPtrs out) {
+                                                     const int64_t *__restrict__ list, F scale, OutPtrs out) {
     constexpr int K = 13 + 4 * NS;
     constexpr int kCells = BX * BY * BZ;
     __shared__ double acc[kCells * K];
@@ -2211,7 +2232,8 @@ __global__ void __launch_bounds__(kBlock) k_m1_brick(Method1Args a, M1Bricks B, 
     };
     const int64_t e0 = offsets[b], e1 = offsets[b + 1];
     for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        const M1Tet T = m1_tet(a, list[e]);
+        const int64_t item = list[e];
+        const M1Tet T = m1_tet(a, item >> 16);
         const M1Fields<F, NS> f = m1_fields<F, VEL, NS>(a, T);
         if (T.kind == 1) {
             const int l = local(T.cell);
@@ -2219,7 +2241,8 @@ __global__ void __launch_bounds__(kBlock) k_m1_brick(Method1Args a, M1Bricks B, 
             continue;
         }
         const double w = T.vol / (double)T.np;
-        for (int64_t i = 0; i < T.np; ++i) {
+        const int64_t i0 = (item & 0xffff) * kM1Chunk, i1 = min(T.np, i0 + kM1Chunk);
+        for (int64_t i = i0; i < i1; ++i) {
             double rel[3];
             int g[3];
             if (!m1_point(a, T, i, rel, g)) continue;
@@ -2879,7 +2902,7 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
 }
 
 template <typename F, bool VEL, int NS>
-void launch_m1_bricks(const Method1Args &ma, const M1Bricks &B, const int64_t *offsets, const int32_t *list,
+void launch_m1_bricks(const Method1Args &ma, const M1Bricks &B, const int64_t *offsets, const int64_t *list,
                       double scale, const OutPtrs &out, hipStream_t s) {
     const dim3 g((unsigned)B.count());
     if (NS > 0)
@@ -2939,19 +2962,19 @@ int launch_method1(dtfe_ctx *ctx, const dtfe_mesh *m, const dtfe_vertex_fields *
     hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(nb)), dim3(kBlock), 0, ctx->stream, count, nb, 0);
     const dim3 gt(grid_for(m->n_cells));
     hipLaunchKernelGGL(k_m1_bin<0>, gt, dim3(kBlock), 0, ctx->stream, ma, B, count, (int64_t *)nullptr,
-                       (int32_t *)nullptr);
+                       (int64_t *)nullptr);
     hipLaunchKernelGGL(k_m1_scan, dim3(1), dim3(1024), 0, ctx->stream, (const int *)count, nb, offsets);
     HIP_TRY(hipGetLastError());
     int64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, offsets + nb, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if ((rc = ensure(ctx, ctx->m1_list, (size_t)std::max<int64_t>(total, 1) * sizeof(int32_t)))) return rc;
+    if ((rc = ensure(ctx, ctx->m1_list, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)))) return rc;
     HIP_TRY(hipMemcpyAsync(cursor, offsets, (size_t)(nb + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice,
                            ctx->stream));
     hipLaunchKernelGGL(k_m1_bin<1>, gt, dim3(kBlock), 0, ctx->stream, ma, B, (int *)nullptr, cursor,
-                       (int32_t *)ctx->m1_list.p);
+                       (int64_t *)ctx->m1_list.p);
     const double scale = 1.0 / ma.grid_volume;
-    const int32_t *list = (const int32_t *)ctx->m1_list.p;
+    const int64_t *list = (const int64_t *)ctx->m1_list.p;
     const bool vel = (fields & (DTFE_FIELD_VELOCITY | kGradUsers)) != 0;
     if (f64) {
         if (vel) {
