@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--no-method1", action="store_true", help="skip the averaging-method-1 timing")
     ap.add_argument("--adjoint-grid", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--workload", choices=["velocity256", "scalars512"], default="velocity256",
+                    help="velocity256: the headline (128^3 particles, 256^3 grid x 8 MC samples); scalars512: "
+                         "256^3 particles, density + 3 scalars + their gradients at the 512^3 cell centres, mixed "
+                         "against all-FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -124,6 +128,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe, max(1, args.steps))
+        return
+    if args.workload == "scalars512":
+        scalars_workload(args)
         return
     import torch
     from dtfe_amd import api, mesh as M
@@ -328,6 +335,122 @@ def fwd_adjoint(args, dm, w, vel, prep, box):
             "workload": f"same mesh; {g}^3 cell-centre density + velocity divergence, FP32 fields; adjoint "
                         f"to positions, weights and velocities from the forward's located cells (loss upstream "
                         f"terms computed in torch)"}
+
+
+SCALAR_FIELDS = ["density", "scalar", "scalar_gradient"]
+
+
+def scalars_workload(args):
+    """256^3 clustered particles (L = 400, 20,000 blobs, sigma in [0.2, 2],
+    density contrast ~1e6) with 3 scalars smoothed by blob: density, scalars
+    and scalar gradients at the cell centres of a 512^3 grid, FP32 fields with
+    FP64 geometry against the all-FP64 path on the same mesh."""
+    import torch
+    from dtfe_amd import api, mesh as M, synthetic as S
+    ps = S.clustered_scalars(n_side=args.n_side if args.n_side != 128 else 256)
+    log(f"particles: {ps.n}")
+    prep = M.prepare_particles(ps)
+    log(f"periodic images: {prep.pos.shape[0]}")
+    cache = args.mesh_cache
+    if cache and os.path.exists(cache):
+        z = np.load(cache)
+        m = M.Mesh(pos=z["pos"], cells=z["cells"], nbrs=z["nbrs"], point_of_vertex=z["pov"],
+                   vertex_of_point=z["vop"], backend=str(z["backend"]), seconds=float(z["seconds"]))
+    else:
+        t0 = time.perf_counter()
+        # the exact tessellator reports its progress every 2M points
+        os.environ.setdefault("DTFE_TESS_PROGRESS", "2000000")
+        m = M.tessellate(prep.pos, backend=args.tess)
+        log(f"tessellation ({m.backend}): {m.n_vertices} vertices, {m.n_cells} cells, {time.perf_counter() - t0:.1f} s")
+        if cache:
+            np.savez(cache, pos=m.pos, cells=m.cells, nbrs=m.nbrs, pov=m.point_of_vertex, vop=m.vertex_of_point,
+                     backend=m.backend, seconds=m.seconds)
+    dev = torch.device("cuda:0")
+    g = args.grid if args.grid != 256 else 512
+    spec = api.GridSpec(prep.region, (g, g, g))
+    dm = api.DeviceMesh.from_mesh(m, dev)
+    ctx = api.context(dev)
+    w_np, s_np = M.vertex_data(m, prep.weight), M.vertex_data(m, prep.scalar)
+    res = {}
+    outs = {}
+    for name, dt in (("mixed", torch.float32), ("fp64", torch.float64)):
+        log(f"{name} path")
+        w = torch.from_numpy(w_np).to(dev, dt)
+        sc = torch.from_numpy(s_np).to(dev, dt)
+
+        def step():
+            rho = api.vertex_density(dm, w, prep.rho_bar)
+            return api.sample(dm, rho, None, sc, SCALAR_FIELDS, spec, method=0)
+
+        for _ in range(max(1, args.warmup)):
+            out = step()
+        torch.cuda.synchronize()
+        k = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = step()
+            k.append(ctx.last_timings())
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res[name] = {"ms_per_step": ms, "sample_kernel_ms": float(np.mean([x[0] for x in k])),
+                     "vertex_density_ms": float(np.mean([x[3] for x in k]))}
+        outs[name] = {f: v.double().cpu().numpy() for f, v in out.items()}
+        del out, w, sc
+        torch.cuda.empty_cache()
+    d32, d64 = outs["mixed"]["density"], outs["fp64"]["density"]
+    nz = d64 > 0
+    err = {"density_max_rel": float(np.max(np.abs(d32[nz] - d64[nz]) / d64[nz])),
+           "density_contrast": float(d64[nz].max() / d64[nz].min())}
+    for f in ("scalar", "scalar_gradient"):
+        a, b = outs["mixed"][f], outs["fp64"][f]
+        err[f + "_max_abs_over_max"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    n_cells = g ** 3
+    log("CPU oracle on one slab")
+    cpu = None if args.no_cpu else cpu_check_scalars(args, prep, m, spec, outs, g)
+    res_line = {
+        "metric": f"DTFE grid samples/s ({g}^3 cell centres, density + 3 scalars + gradients)",
+        "value": n_cells / (res["mixed"]["ms_per_step"] * 1e-3),
+        "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["mixed"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"clustered_scalars N={round(ps.n ** (1 / 3))}^3 L=400 20000 blobs sigma [0.2,2], "
+                               f"3 scalars; {g}^3 cell centres",
+                   "n_particles": ps.n, "n_vertices": m.n_vertices, "n_tetrahedra": m.n_cells,
+                   "tessellation": m.backend, "tessellation_s": round(m.seconds, 3)},
+        "mixed": res["mixed"], "fp64": res["fp64"],
+        "mixed_faster_than_fp64": res["mixed"]["ms_per_step"] < res["fp64"]["ms_per_step"],
+        "mixed_vs_fp64": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res_line), flush=True)
+
+
+def cpu_check_scalars(args, prep, m, spec, outs, g):
+    """The CPU oracle (FP64) on one x-slab of the grid, timed on one thread:
+    its cell-centre fields against both GPU paths."""
+    from oracle import oracle as O
+    from dtfe_amd import mesh as M
+    box = spec.box
+    dx = (box[1] - box[0]) / g
+    i = g // 2
+    sub = (box[0] + i * dx, box[0] + (i + 1) * dx, box[2], box[3], box[4], box[5])
+    w = M.vertex_data(m, prep.weight)
+    sc = M.vertex_data(m, prep.scalar)
+    t0 = time.perf_counter()
+    rho, _, _ = O.vertex_density(m, w, prep.rho_bar)
+    t_vd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref = O.sample(m, rho, None, sc, fields=tuple(SCALAR_FIELDS), kind="grid", box=sub, n=(1, g, g), method=0)
+    t = time.perf_counter() - t0
+    rows = slice(i * g * g, (i + 1) * g * g)
+    errs = {}
+    for name in ("mixed", "fp64"):
+        for f in SCALAR_FIELDS:
+            got, exp = outs[name][f][rows], ref[f]
+            errs[f"{name}_{f}_max_abs_over_max"] = float(np.max(np.abs(got - exp)) / np.max(np.abs(exp)))
+    return {"value": g * g / t, "unit": "samples/s", "cores": 1, "kind": "port",
+            "sample": f"one x-slab (1 x {g} x {g} cell centres) of the grid, FP64 oracle on one thread",
+            "seconds": t, "vertex_density_seconds_one_thread": t_vd, "oracle_vs_gpu": errs}
 
 
 def front_end(ps, prep):

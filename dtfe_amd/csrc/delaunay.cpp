@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -559,10 +561,17 @@ int dtfe_tess_build(int64_t n, const double *pts, dtfe_tess **out) {
         return 2;
     }
     D.init(i0, i1, i2, i3);
+    // DTFE_TESS_PROGRESS=k: a line on stderr after every k inserted points
+    int64_t every = 0;
+    if (const char *e = std::getenv("DTFE_TESS_PROGRESS")) every = std::atoll(e);
     for (size_t j = 0; j < order.size(); ++j) {
         int p = order[j];
         if (p == i0 || p == i1 || p == i2 || p == i3) continue;
         D.insert(p);
+        if (every > 0 && (int64_t)(j + 1) % every == 0) {
+            std::fprintf(stderr, "[tessellation] %zu of %zu points inserted\n", j + 1, order.size());
+            std::fflush(stderr);
+        }
     }
     // finite tetrahedra
     std::vector<int> live(D.tets.size(), 0);

@@ -1330,10 +1330,12 @@ __global__ void __launch_bounds__(kLanesBlock, DTFE_LANES_MIN_WAVES) k_sample_mc
         for (int q = 0; q < 3; ++q) acc.v[q] = lane_sum<LPR>(acc.v[q]);
 #pragma unroll
         for (int q = 0; q < 9; ++q) acc.vg[q] = lane_sum<LPR>(acc.vg[q]);
+        if (NS > 0) {
 #pragma unroll
-        for (int q = 0; q < (NS > 0 ? NS : 1); ++q) acc.s[q] = lane_sum<LPR>(acc.s[q]);
+            for (int q = 0; q < (NS > 0 ? NS : 1); ++q) acc.s[q] = lane_sum<LPR>(acc.s[q]);
 #pragma unroll
-        for (int q = 0; q < 3 * (NS > 0 ? NS : 1); ++q) acc.sg[q] = lane_sum<LPR>(acc.sg[q]);
+            for (int q = 0; q < 3 * (NS > 0 ? NS : 1); ++q) acc.sg[q] = lane_sum<LPR>(acc.sg[q]);
+        }
         F sd = 0;
         if (want_std) {
             // the reference's two-pass standard deviation over the M values
@@ -2210,6 +2212,8 @@ __global__ void __launch_bounds__(1024) k_m1_scan(const int *__restrict__ count,
 // One workgroup per brick: every tetrahedron of its list adds its
 // contributions inside the brick to LDS sums (FP64), which are then divided
 // by the grid cell volume and written as rows with the derived fields.
+// Bricks run in grid order: neighbouring workgroups share tetrahedra in L2
+// (ordering them by list length, longest first, was slower).
 template <typename F, bool VEL, int NS, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(kBlock) k_m1_brick(Method1Args a, M1Bricks B, const int64_t *__restrict__ offsets,
                                                      const int64_t *__restrict__ list, F scale, OutPtrs out) {
