@@ -51,10 +51,11 @@ def parse():
     ap.add_argument("--no-method1", action="store_true", help="skip the averaging-method-1 timing")
     ap.add_argument("--adjoint-grid", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--workload", choices=["velocity256", "scalars512"], default="velocity256",
+    ap.add_argument("--workload", choices=["velocity256", "scalars512", "cone4m"], default="velocity256",
                     help="velocity256: the headline (128^3 particles, 256^3 grid x 8 MC samples); scalars512: "
                          "256^3 particles, density + 3 scalars + their gradients at the 512^3 cell centres, mixed "
-                         "against all-FP64")
+                         "against all-FP64; cone4m: 4e6 non-periodic particles, a 256x128x128 redshift cone and "
+                         "1e6 user points, 4 MC samples each, forward and adjoint to weights and velocities")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of this many MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -131,6 +132,9 @@ def main():
         return
     if args.workload == "scalars512":
         scalars_workload(args)
+        return
+    if args.workload == "cone4m":
+        cone_workload(args)
         return
     import torch
     from dtfe_amd import api, mesh as M
@@ -423,6 +427,104 @@ def scalars_workload(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(res_line), flush=True)
+
+
+CONE = (10.0, 400.0, 15.0, 75.0, 15.0, 75.0)  # r, theta, psi ranges (degrees), observer at the origin
+
+
+def cone_points(n, seed):
+    """n points uniform in the volume of CONE (r^3-, cos(theta)- and psi-uniform)
+    and, per point, a cube with the volume of the cone grid cell there."""
+    rng = np.random.default_rng(seed)
+    r0, r1, t0, t1, p0, p1 = CONE
+    r = np.cbrt(rng.uniform(r0 ** 3, r1 ** 3, n))
+    th = np.arccos(rng.uniform(np.cos(np.radians(t1)), np.cos(np.radians(t0)), n))
+    ps = np.radians(rng.uniform(p0, p1, n))
+    pts = np.stack([r * np.sin(th) * np.cos(ps), r * np.sin(th) * np.sin(ps), r * np.cos(th)], 1)
+    dr, dth, dps = (r1 - r0) / 256, np.radians(t1 - t0) / 128, np.radians(p1 - p0) / 128
+    side = np.cbrt(dr * (r * dth) * (r * np.sin(th) * dps))
+    return pts, np.repeat(side[:, None], 3, 1)
+
+
+def cone_workload(args):
+    """4e6 clustered particles in a non-periodic box of side 800 (4000
+    blobs), weights exp(-r/300) from the observer at the corner: density and
+    velocity averaged with 4 Monte-Carlo samples over the cells of a
+    256 x 128 x 128 redshift cone (r in [10, 400], 60 x 60 degrees) and over
+    1e6 user points uniform in the cone; then the adjoint to the weights and
+    velocities. FP32 fields."""
+    import torch
+    from dtfe_amd import api, mesh as M, synthetic as S
+    ps = S.cone_survey()
+    log(f"particles: {ps.n}")
+    prep = M.prepare_particles(ps)
+    cache = args.mesh_cache
+    if cache and os.path.exists(cache):
+        z = np.load(cache)
+        m = M.Mesh(pos=z["pos"], cells=z["cells"], nbrs=z["nbrs"], point_of_vertex=z["pov"],
+                   vertex_of_point=z["vop"], backend=str(z["backend"]), seconds=float(z["seconds"]))
+    else:
+        os.environ.setdefault("DTFE_TESS_PROGRESS", "1000000")
+        m = M.tessellate(prep.pos, backend=args.tess)
+        log(f"tessellation ({m.backend}): {m.n_vertices} vertices, {m.n_cells} cells, {m.seconds:.1f} s")
+        if cache:
+            np.savez(cache, pos=m.pos, cells=m.cells, nbrs=m.nbrs, pov=m.point_of_vertex, vop=m.vertex_of_point,
+                     backend=m.backend, seconds=m.seconds)
+    dev = torch.device("cuda:0")
+    dm = api.DeviceMesh.from_mesh(m, dev)
+    w = torch.from_numpy(M.vertex_data(m, prep.weight)).to(dev, torch.float32)
+    vel = torch.from_numpy(M.vertex_data(m, prep.vel)).to(dev, torch.float32)
+    pts, dl = cone_points(1_000_000, 11)
+    specs = {"cone": api.ConeSpec(CONE, (256, 128, 128), (0.0, 0.0, 0.0)),
+             "points": api.PointsSpec(torch.from_numpy(pts).to(dev), torch.from_numpy(dl).to(dev), ps.box)}
+    fields = ["density", "velocity"]
+    seed = 424242
+    res = {}
+    for name, spec in specs.items():
+        log(f"{name}")
+
+        def forward():
+            rho = api.vertex_density(dm, w, prep.rho_bar)
+            return rho, api.sample(dm, rho, vel, None, fields, spec, method=2, samples=4, seed=seed,
+                                   want_cells=True)
+
+        rho, out = forward()
+        gen = torch.Generator(device="cpu").manual_seed(5)
+        up = {f: torch.randn(tuple(out[f].shape), generator=gen).to(dev) for f in fields}
+
+        def both():
+            rho, out = forward()
+            return api.adjoint(dm, rho, w, vel, prep.rho_bar, spec, up, method=2, samples=4, seed=seed,
+                               want=("weight", "velocity"), start_cells=out["located_cell"])
+
+        def timed(fn, k):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / k
+
+        k = max(3, args.steps)
+        tf = timed(forward, k)
+        tfa = timed(both, k)
+        rows = out["density"].shape[0]
+        res[name] = {"rows": rows, "samples": rows * 4, "forward_ms": tf, "forward_adjoint_ms": tfa,
+                     "fwd_adjoint_ratio": tfa / tf, "forward_samples_per_s": rows * 4 / (tf * 1e-3)}
+    line = {
+        "metric": "DTFE grid samples/s (redshift cone 256x128x128 x 4 MC samples, density + velocity)",
+        "value": res["cone"]["forward_samples_per_s"], "unit": "samples/s", "n_gpus": 1,
+        "steps": max(3, args.steps), "warmup": 1, "ms_per_step": res["cone"]["forward_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cone_survey N=4e6 L=800 non-periodic, weights exp(-r/300); cone r [10,400] "
+                               "theta, psi [15,75] deg, 256x128x128 x 4 MC; 1e6 user points x 4 MC",
+                   "n_particles": ps.n, "n_vertices": m.n_vertices, "n_tetrahedra": m.n_cells,
+                   "tessellation_s": round(m.seconds, 3)},
+        "cone": res["cone"], "points": res["points"],
+        "adjoint_to": ["weight", "velocity"],
+    }
+    print(json.dumps(line), flush=True)
 
 
 def cpu_check_scalars(args, prep, m, spec, outs, g):

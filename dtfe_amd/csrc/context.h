@@ -26,7 +26,7 @@ struct dtfe_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
-    DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, plan, flags, rows, mass, mom,
+    DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, flags, rows, mass, mom,
         defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, adj_acc, centre, records, m1_qr, m1_bins, m1_list,
         front;  // scratch of the front-end calls (counts, offsets, partial sums, flags)
     int last_deferred = 0;
@@ -38,11 +38,18 @@ struct dtfe_ctx {
     // 0 sample main kernel, 1 exact kernel, 2 seed grid, 3 vertex density, 4 centre location
     double last_ms[5] = {0, 0, 0, 0, 0};
     // MC plan cache
-    bool plan_valid = false;
-    uint32_t plan_seed = 0;
-    int64_t plan_wpr = 0, plan_rows = 0, plan_rpb = 0;
+    // Monte-Carlo plans (MT19937 states per workgroup), the two most recent
+    // (seed, words per row, rows, rows per workgroup): a forward and its
+    // adjoint split the rows differently and take turns
+    struct Plan {
+        DevBuf buf;
+        bool valid = false;
+        uint32_t seed = 0;
+        int64_t wpr = 0, rows = 0, rpb = 0;
+        uint64_t used = 0;
+    } plans[2];
+    uint64_t plan_clock = 0;
     std::vector<uint32_t> plan_host;
-    double last_plan_ms = 0.0;
 };
 
 namespace {

@@ -866,6 +866,12 @@ struct GlobalSink {
 // 64 scattered cache lines.
 constexpr int kSinkSlots = 512;
 constexpr int kSinkVals = 7;
+// probes before a vertex goes straight to HBM: rows spread over many
+// vertices (the outer cells of a cone) fill the table, and long probe
+// sequences cost more than the HBM atomic they avoid
+#ifndef DTFE_SINK_PROBES
+#define DTFE_SINK_PROBES 16
+#endif
 
 struct LdsSink {
     int *keys;      // [kSinkSlots], -1 = free
@@ -880,7 +886,7 @@ struct LdsSink {
     template <typename F>
     __device__ __forceinline__ void add(const SampleArgs &a, int v, const VertexGrad &d) const {
         unsigned slot = ((unsigned)v * 2654435761u) >> 23;  // 9 bits
-        for (int probe = 0; probe < 16; ++probe) {
+        for (int probe = 0; probe < DTFE_SINK_PROBES; ++probe) {
             const int k = atomicCAS(&keys[slot], -1, v);
             if (k == -1 || k == v) {
                 double *o = vals + slot * kSinkVals;
@@ -1435,6 +1441,44 @@ __global__ void DTFE_SAMPLE_BOUNDS k_sample_mc(SampleArgs a) {
         __syncthreads();
         for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
             if (!run_row<F, VEL, NS, false, ADJ>(a, row, words)) defer_row(a, row, words);
+    }
+}
+
+// Backward of Monte-Carlo rows: as k_sample_mc (the workgroup regenerates
+// its rows' MT words in the LDS ring), with the per-vertex gradients of each
+// round of rows merged in an LDS table before the HBM atomics. Many rows
+// meet the same few vertices where grid cells are much smaller than the
+// tetrahedra (the inner cells of a redshift cone), and one atomic per vertex
+// and round replaces one per row there.
+template <typename F, bool VEL, int KIND>
+__global__ void DTFE_SAMPLE_BOUNDS k_adjoint_mc(SampleArgs a) {
+    extern __shared__ uint32_t ring[];
+    __shared__ int keys[kSinkSlots];
+    __shared__ double vals[kSinkSlots * kSinkVals];
+    const LdsSink sink{keys, vals};
+    const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
+    if (row0 >= a.n_rows) return;
+    const int64_t row_end = min(row0 + a.rows_per_block, a.n_rows);
+    const int64_t wpr = 3 * (int64_t)a.M;
+    const int64_t first_block = (wpr * row0) / MT_N;
+    const int64_t base = first_block - 1;
+    for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
+    RingWords<> words{ring, base * MT_N, (uint32_t)a.ring};
+    int64_t have = first_block - 1;
+    for (int64_t r0 = row0; r0 < row_end; r0 += a.rows_per_round) {
+        const int64_t r1 = min(r0 + a.rows_per_round, row_end);
+        const int64_t need = (wpr * r1 - 1) / MT_N;
+        while (have < need) {
+            mt_generate_block(ring, have + 1, base, a.ring);
+            ++have;
+        }
+        sink.clear();
+        __syncthreads();
+        for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
+            if (!adjoint_row<F, VEL, false, RingWords<>, LdsSink, KIND>(a, row, words, sink)) defer_row(a, row, words);
+        __syncthreads();
+        sink.template flush<F>(a);
+        __syncthreads();
     }
 }
 
@@ -2550,7 +2594,13 @@ void launch_sample(const SampleArgs &a, int which, int mc, int grid, size_t lds,
         hipLaunchKernelGGL((k_sample_exact<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), 0, s, a, n_def);
     else if (which == 2)
         hipLaunchKernelGGL((k_sample_exact_all<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), mc ? lds : 0, s, a);
-    else if (mc)
+    else if (mc && ADJ) {
+        switch (a.kind) {
+            case DTFE_SAMPLE_GRID: hipLaunchKernelGGL((k_adjoint_mc<F, VEL, DTFE_SAMPLE_GRID>), dim3(grid), dim3(kBlock), lds, s, a); break;
+            case DTFE_SAMPLE_CONE: hipLaunchKernelGGL((k_adjoint_mc<F, VEL, DTFE_SAMPLE_CONE>), dim3(grid), dim3(kBlock), lds, s, a); break;
+            default: hipLaunchKernelGGL((k_adjoint_mc<F, VEL, DTFE_SAMPLE_POINTS>), dim3(grid), dim3(kBlock), lds, s, a); break;
+        }
+    } else if (mc)
         hipLaunchKernelGGL((k_sample_mc<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), lds, s, a);
     else if (ADJ) {
         const BrickGrid bg = brick_grid(a.n);
@@ -2777,8 +2827,10 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     if (is_mc) {
         const int64_t wpr = 3 * (int64_t)a.M;
         int rpr = kBlock;
-        // ring must hold every MT block one round touches plus the previous one
-        while (rpr > 1 && ((wpr * rpr + MT_N - 1) / MT_N + 2) * MT_N * 4 > 48 * 1024) rpr /= 2;
+        // ring must hold every MT block one round touches plus the previous one;
+        // the adjoint's workgroups also hold a 30 KB LDS table
+        const int64_t ring_cap = adj ? 32 * 1024 : 48 * 1024;
+        while (rpr > 1 && ((wpr * rpr + MT_N - 1) / MT_N + 2) * MT_N * 4 > ring_cap) rpr /= 2;
         a.rows_per_round = rpr;
         a.ring = (int)((wpr * rpr + MT_N - 1) / MT_N + 2);
         lds = (size_t)a.ring * MT_N * sizeof(uint32_t);
@@ -2797,22 +2849,28 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
         uint64_t s = mc->seed;
         if (mc->part_no >= 0) s += 1000ull * (uint64_t)mc->part_no;
         uint32_t seed32 = (uint32_t)s;
-        if (!(ctx->plan_valid && ctx->plan_seed == seed32 && ctx->plan_wpr == wpr && ctx->plan_rows == n_rows &&
-              ctx->plan_rpb == a.rows_per_block)) {
+        dtfe_ctx::Plan *plan = nullptr;
+        for (auto &pl : ctx->plans)
+            if (pl.valid && pl.seed == seed32 && pl.wpr == wpr && pl.rows == n_rows && pl.rpb == a.rows_per_block)
+                plan = &pl;
+        if (!plan) {
+            plan = ctx->plans[0].used <= ctx->plans[1].used ? &ctx->plans[0] : &ctx->plans[1];
+            plan->valid = false;
             std::vector<int64_t> first(nblk);
             for (int64_t b = 0; b < nblk; ++b) first[b] = (wpr * (b * a.rows_per_block)) / MT_N;
             mt_build_plan(seed32, first, ctx->plan_host);
-            if ((rc = ensure(ctx, ctx->plan, ctx->plan_host.size() * sizeof(uint32_t)))) return rc;
-            HIP_TRY(hipMemcpyAsync(ctx->plan.p, ctx->plan_host.data(), ctx->plan_host.size() * sizeof(uint32_t),
+            if ((rc = ensure(ctx, plan->buf, ctx->plan_host.size() * sizeof(uint32_t)))) return rc;
+            HIP_TRY(hipMemcpyAsync(plan->buf.p, ctx->plan_host.data(), ctx->plan_host.size() * sizeof(uint32_t),
                                    hipMemcpyHostToDevice, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
-            ctx->plan_valid = true;
-            ctx->plan_seed = seed32;
-            ctx->plan_wpr = wpr;
-            ctx->plan_rows = n_rows;
-            ctx->plan_rpb = a.rows_per_block;
+            plan->valid = true;
+            plan->seed = seed32;
+            plan->wpr = wpr;
+            plan->rows = n_rows;
+            plan->rpb = a.rows_per_block;
         }
-        a.plan = (const uint32_t *)ctx->plan.p;
+        plan->used = ++ctx->plan_clock;
+        a.plan = (const uint32_t *)plan->buf.p;
     } else {
         grid = grid_for(n_rows);
     }
@@ -3042,12 +3100,14 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->W, &ctx->hull, &ctx->seed_b, &ctx->bbox_part,
-                      &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->plan, &ctx->flags,
+                      &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
                       &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->m1_qr,
                       &ctx->m1_bins, &ctx->m1_list, &ctx->front};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
+    for (auto &pl : ctx->plans)
+        if (pl.buf.p) (void)hipFree(pl.buf.p);
     for (int i = 0; i < 9; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
