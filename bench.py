@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint / forward ratio")
     ap.add_argument("--no-method1", action="store_true", help="skip the averaging-method-1 timing")
+    ap.add_argument("--no-sph", action="store_true", help="skip the SPH timing")
+    ap.add_argument("--sph-neighbours", type=int, default=32)
     ap.add_argument("--adjoint-grid", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--workload", choices=["velocity256", "scalars512", "cone4m"], default="velocity256",
@@ -248,6 +250,10 @@ def main():
               "points_per_grid_cell": args.samples,
               "workload": "same mesh, grid and fields; averaging method 1 (Sobol points in the tetrahedra)"}
 
+    sph = None
+    if not args.no_sph:
+        sph = sph_timing(args, ps, prep, spec, dt, dev)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, ps, prep, m, box, n, mc_seed)
@@ -286,11 +292,43 @@ def main():
         "fp64_path": fp64,
         "front_end_ms": fe,
         "method1": m1,
+        "sph": sph,
         "fwd_adjoint_ratio": None if fa is None else fa["ratio"],
         "fwd_adjoint": fa,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def sph_timing(args, ps, prep, spec, dt, dev):
+    """SPH interpolation of the same padded particles (the reference pads
+    for SPH as for DTFE) to the same grid, density and velocity: one call
+    builds the tree, finds every particle's and grid point's neighbours and
+    sums both kernel halves."""
+    import torch
+    from dtfe_amd import api
+    log("sph")
+    iof = prep.image_of
+    pos = torch.from_numpy(prep.pos).to(dev, dt)
+    w = torch.from_numpy(ps.weight[iof]).to(dev, dt)
+    vel = torch.from_numpy(ps.vel[iof]).to(dev, dt)
+    N = args.sph_neighbours
+    for _ in range(2):
+        api.sph(pos, w, vel, None, spec, N, prep.rho_bar, ("density", "velocity"))
+    torch.cuda.synchronize()
+    parts = []
+    t0 = time.perf_counter()
+    for _ in range(3):
+        api.sph(pos, w, vel, None, spec, N, prep.rho_bar, ("density", "velocity"))
+        parts.append(api.last_sph_timings(dev))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 3
+    parts = np.mean(np.array(parts), axis=0)
+    rows = int(np.prod(spec.n))
+    return {"ms_per_call": ms, "tree_ms": float(parts[0]), "particle_pass_ms": float(parts[1]),
+            "sample_pass_ms": float(parts[2]), "neighbours": N, "particles": int(pos.shape[0]),
+            "grid_points": rows, "grid_points_per_s": rows / (ms * 1e-3),
+            "workload": "same padded particles and grid; SPH density and velocity"}
 
 
 def fwd_adjoint(args, dm, w, vel, prep, box):

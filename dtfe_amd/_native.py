@@ -51,6 +51,7 @@ EXPORTED = [
     "dtfe_hip_version", "dtfe_hip_last_timings", "dtfe_hip_last_deferred", "dtfe_hip_walk_failures",
     "dtfe_hip_last_walk_stats", "dtfe_hip_adjoint", "dtfe_hip_locator_bytes", "dtfe_hip_build_locator",
     "dtfe_hip_periodic_images", "dtfe_hip_average_density", "dtfe_hip_gather_rows", "dtfe_hip_scatter_add_rows",
+    "dtfe_hip_sph", "dtfe_hip_last_sph_timings",
 ]
 
 CELL_RECORD_BYTES = 128
@@ -141,6 +142,9 @@ def load(path: str = LIB_PATH):
         lib.dtfe_hip_average_density.argtypes = [vp, i64, i32, vp, C.POINTER(C.c_double)]
         lib.dtfe_hip_gather_rows.argtypes = [vp, i64, vp, i64, i32, i32, vp, vp]
         lib.dtfe_hip_scatter_add_rows.argtypes = [vp, i64, vp, i64, i32, i32, vp, vp]
+        lib.dtfe_hip_sph.argtypes = [vp, i64, i32, vp, vp, vp, vp, i32, i32, i32, C.c_uint, C.POINTER(Sampling),
+                                     C.c_double, C.POINTER(Outputs)]
+        lib.dtfe_hip_last_sph_timings.argtypes = [vp, C.POINTER(C.c_double * 3)]
         lib.dtfe_hip_adjoint.argtypes = [vp, C.POINTER(Mesh), C.POINTER(VertexFields), vp, C.c_double,
                                          C.POINTER(Sampling), C.POINTER(MC), C.c_uint, C.POINTER(Upstream),
                                          C.POINTER(VertexGrads)]

@@ -241,6 +241,66 @@ def deposit(scheme: str, pos: torch.Tensor, weight: torch.Tensor, vel: Optional[
     return out
 
 
+def sph(pos: torch.Tensor, weight: torch.Tensor, vel: Optional[torch.Tensor], scalar: Optional[torch.Tensor],
+        spec, neighbours: int, rho_bar: float, fields=("density", "velocity"), extensive: bool = False
+        ) -> Dict[str, torch.Tensor]:
+    """SPH interpolation (SPH_interpolation, src/SPH_interpolation.cc:50-474) of
+    particles pos [np, 3], weight [np], vel [np, 3], scalar [np, S] (one dtype,
+    the reference's Real) at the rows of a GridSpec, ConeSpec (limits used as
+    given, as the reference's SPH path does) or PointsSpec (every point gives
+    a row). Fields: density, velocity, scalar."""
+    dev = pos.device
+    ctx = context(dev)
+    dtype = weight.dtype
+    n_scalar = 0 if scalar is None else int(scalar.shape[1])
+    keep = []
+    sp = N.Sampling()
+    if isinstance(spec, GridSpec):
+        sp.kind = N.SAMPLE_GRID
+        sp.box[:] = [float(x) for x in spec.box]
+        sp.n[:] = [int(x) for x in spec.n]
+        n_rows = int(np.prod(spec.n))
+    elif isinstance(spec, ConeSpec):
+        sp.kind = N.SAMPLE_CONE
+        sp.cone[:] = [float(x) for x in spec.cone]
+        sp.n[:] = [int(x) for x in spec.n]
+        sp.origin[:] = [float(x) for x in spec.origin]
+        n_rows = int(np.prod(spec.n))
+    elif isinstance(spec, PointsSpec):
+        sp.kind = N.SAMPLE_POINTS
+        pts = spec.points.to(torch.float64).contiguous()
+        keep.append(pts)
+        sp.n_points = pts.shape[0]
+        sp.points = pts.data_ptr()
+        n_rows = int(pts.shape[0])
+    else:
+        raise TypeError("spec must be GridSpec, ConeSpec or PointsSpec")
+    out = {}
+    bits = 0
+    o = N.Outputs()
+    for f in fields:
+        bits |= FIELD_BITS[f]
+        k = components(f, n_scalar)
+        out[f] = torch.empty((n_rows, k) if k != 1 else (n_rows,), dtype=dtype, device=dev)
+        setattr(o, f, out[f].data_ptr())
+    pos, weight = pos.to(dtype).contiguous(), weight.contiguous()
+    vel = None if vel is None else vel.to(dtype).contiguous()
+    scalar = None if scalar is None else scalar.to(dtype).contiguous()
+    rc = ctx.lib.dtfe_hip_sph(ctx.h, pos.shape[0], _dtype_code(dtype), _ptr(pos), _ptr(weight), _ptr(vel),
+                              _ptr(scalar), n_scalar, int(neighbours), int(bool(extensive)), bits, C.byref(sp),
+                              float(rho_bar), C.byref(o))
+    ctx.check(rc)
+    return out
+
+
+def last_sph_timings(device: torch.device):
+    """Milliseconds of the last SPH call on this device: tree, particles, samples."""
+    ctx = context(device)
+    ms = (C.c_double * 3)()
+    ctx.check(ctx.lib.dtfe_hip_last_sph_timings(ctx.h, C.byref(ms)))
+    return list(ms)
+
+
 UPSTREAM_FIELDS = ("density", "velocity", "velocity_gradient", "velocity_divergence", "velocity_shear",
                    "velocity_vorticity")
 

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libdtfe_hip.so")
 HOST_LIB = os.path.join(HERE, "libdtfe_host.so")
 HOST_SOURCES = ["delaunay.cpp"]
 
-HIP_SOURCES = ["dtfe_hip.hip", "front_end.hip"]
+HIP_SOURCES = ["dtfe_hip.hip", "front_end.hip", "sph.hip"]
 HIP_HEADERS = ["predicates.h", "mt19937.h", "fastdiv.h", "context.h"]
 
 HIPCC_FLAGS = [

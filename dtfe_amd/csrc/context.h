@@ -28,7 +28,8 @@ struct dtfe_ctx {
     std::string err;
     DevBuf W, hull, seed_b, bbox_part, status, axes, trig, equi, flags, rows, mass, mom,
         defer_rows, defer_words, adj_rho, adj_dW, adj_dW2, adj_partial, adj_acc, centre, records, m1_qr, m1_bins, m1_list,
-        front;  // scratch of the front-end calls (counts, offsets, partial sums, flags)
+        front,  // scratch of the front-end calls (counts, offsets, partial sums, flags)
+        sph;    // SPH: sorted particles, tree boxes, per-particle smoothing data, sort scratch
     int last_deferred = 0;
     int64_t m1_points = 0;  // quasi-random points held in m1_qr (averaging method 1)
     int64_t last_walk[4] = {0, 0, 0, 0};  // centre steps, sample steps, rows, samples
@@ -37,6 +38,10 @@ struct dtfe_ctx {
     bool ev_pending[4] = {false, false, false, false};
     // 0 sample main kernel, 1 exact kernel, 2 seed grid, 3 vertex density, 4 centre location
     double last_ms[5] = {0, 0, 0, 0, 0};
+    // SPH: events around the tree build, the particle pass and the sample pass
+    hipEvent_t sph_ev[4] = {};
+    bool sph_pending = false;
+    double sph_ms[3] = {0, 0, 0};
     // MC plan cache
     // Monte-Carlo plans (MT19937 states per workgroup), the two most recent
     // (seed, words per row, rows, rows per workgroup): a forward and its

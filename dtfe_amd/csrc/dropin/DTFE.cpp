@@ -7,7 +7,7 @@
 // One partition: the GPU replaces the reference's OpenMP split
 // (src/DTFE.cpp:287-385), so results equal the reference's single-thread run.
 // Not provided: --poisson / --randomSample input generation (src/random.cc),
-// --partition serial chunking, NGP and SPH;
+// --partition serial chunking and NGP;
 // asking for them is a fatal error, as an invalid option is in the reference.
 #include "DTFE.h"
 
@@ -470,6 +470,62 @@ void deposit(std::vector<Particle_data> *p, User_options &o, Quantities *aQ, int
     if (o.aField.velocity) download(aQ->velocity, d_v, ncell);
 }
 
+// SPH_interpolation (src/SPH_interpolation.cc:50-474) through dtfe_hip_sph:
+// user points when given, else the redshift cone or the grid over the region
+void sph(std::vector<Particle_data> *p, std::vector<Sample_point> &samples, User_options &o, Quantities *aQ) {
+    const size_t np = p->size();
+    std::vector<Real> pos(3 * np), w(np), vel(3 * np), sc(noScalarComp * np);
+    for (size_t i = 0; i < np; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            pos[3 * i + k] = (*p)[i].pos[k];
+            vel[3 * i + k] = (*p)[i].velocity()[k];
+        }
+        for (size_t k = 0; k < noScalarComp; ++k) sc[noScalarComp * i + k] = (*p)[i].scalar()[k];
+        w[i] = (*p)[i].weight();
+    }
+    p->clear();
+    Ctx ctx(o.device);
+    DevMem d_pos = upload(pos.data(), pos.size()), d_w = upload(w.data(), w.size()),
+           d_vel = upload(vel.data(), vel.size()), d_sc = upload(sc.data(), sc.size());
+    dtfe_sampling smp{};
+    std::vector<double> spts;
+    DevMem d_spts;
+    size_t n_rows;
+    if (!samples.empty()) {
+        smp.kind = DTFE_SAMPLE_POINTS;
+        for (auto &s : samples)
+            for (int k = 0; k < 3; ++k) spts.push_back(s.pos[k]);
+        d_spts = upload(spts.data(), spts.size());
+        smp.n_points = (int64_t)samples.size();
+        smp.points = d_spts.as<double>();
+        n_rows = samples.size();
+    } else {
+        smp.kind = o.redshiftConeOn ? DTFE_SAMPLE_CONE : DTFE_SAMPLE_GRID;
+        for (int i = 0; i < 6; ++i) smp.box[i] = o.region[i];
+        for (int i = 0; i < 3; ++i) smp.n[i] = (int64_t)o.gridSize[i];
+        if (o.redshiftConeOn) {
+            for (int i = 0; i < 6; ++i) smp.cone[i] = o.redshiftCone[i];
+            for (int i = 0; i < 3; ++i) smp.origin[i] = o.originPosition[i];
+        }
+        n_rows = o.gridSize[0] * o.gridSize[1] * o.gridSize[2];
+    }
+    // the density is always computed: velocity and intensive scalars divide by it
+    DevMem d_den(n_rows * sizeof(Real)), d_v(3 * n_rows * sizeof(Real)), d_s(noScalarComp * n_rows * sizeof(Real));
+    dtfe_outputs out{};
+    out.density = d_den.p;
+    out.velocity = d_v.p;
+    out.scalar = d_s.p;
+    unsigned bits = DTFE_FIELD_DENSITY | (o.aField.velocity ? DTFE_FIELD_VELOCITY : 0u) |
+                    (o.aField.scalar ? DTFE_FIELD_SCALAR : 0u);
+    ctx.check(dtfe_hip_sph(ctx.h, (int64_t)np, kDtype, d_pos.p, d_w.p, d_vel.p, d_sc.p, (int)noScalarComp,
+                           o.SPH_neighbors, o.extensive ? 1 : 0, bits, &smp, (double)o.averageDensity, &out),
+              "SPH interpolation");
+    ctx.check(dtfe_hip_synchronize(ctx.h), "SPH interpolation");
+    if (o.aField.density) download(aQ->density, d_den, n_rows);
+    if (o.aField.velocity) download(aQ->velocity, d_v, n_rows);
+    if (o.aField.scalar) download(aQ->scalar, d_s, n_rows);
+}
+
 // interpolate (src/DTFE.cpp:263-281)
 void interpolate(std::vector<Particle_data> *p, std::vector<Sample_point> &samples, User_options &o, Quantities *uQ,
                  Quantities *aQ) {
@@ -477,7 +533,7 @@ void interpolate(std::vector<Particle_data> *p, std::vector<Sample_point> &sampl
     else if (o.CIC) deposit(p, o, aQ, 0);
     else if (o.TSC) deposit(p, o, aQ, 1);
     else if (o.NGP) fatal("the NGP method is not provided by the MI355X drop-in");
-    else if (o.SPH) fatal("the SPH method is not provided by the MI355X drop-in");
+    else if (o.SPH) sph(p, samples, o, aQ);
     else fatal("unknown interpolation method in function 'interpolate'");
 }
 

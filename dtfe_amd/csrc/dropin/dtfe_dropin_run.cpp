@@ -4,7 +4,7 @@
 //   dtfe_dropin_run <options.txt> <particles.bin> [<samples.bin>] <out_prefix>
 //
 // options.txt: "key value..." lines (box, grid, periodic, fields_u, fields_a,
-// method, points, seed, cone, origin, region, CIC, TSC, density0);
+// method, points, seed, cone, origin, region, CIC, TSC, SPH n, extensive, density0);
 // particles.bin: n x 7 doubles (x, y, z, weight, vx, vy, vz);
 // samples.bin: m x 6 doubles (x, y, z, dx, dy, dz).
 // Each output field F of uQuantities/aQuantities goes to <out_prefix>.{u,a}.F
@@ -87,6 +87,8 @@ int main(int argc, char **argv) {
         else if (k == "region") { o.regionOn = true; for (int i = 0; i < 6; ++i) ss >> o.region[i]; }
         else if (k == "CIC") o.CIC = true;
         else if (k == "TSC") o.TSC = true;
+        else if (k == "SPH") { o.SPH = true; ss >> o.SPH_neighbors; }
+        else if (k == "extensive") o.extensive = true;
         else if (k == "density0") ss >> o.averageDensity;
         else if (k == "value_formula") { int v; ss >> v; o.referenceValueFormula = v != 0; }
     }

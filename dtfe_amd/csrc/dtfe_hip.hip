@@ -3103,13 +3103,15 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
                       &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->m1_qr,
-                      &ctx->m1_bins, &ctx->m1_list, &ctx->front};
+                      &ctx->m1_bins, &ctx->m1_list, &ctx->front, &ctx->sph};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &pl : ctx->plans)
         if (pl.buf.p) (void)hipFree(pl.buf.p);
     for (int i = 0; i < 9; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 4; ++i)
+        if (ctx->sph_ev[i]) (void)hipEventDestroy(ctx->sph_ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return DTFE_OK;

@@ -1,0 +1,776 @@
+// SPH interpolation on the device, replacing SPH_interpolation
+// (src/SPH_interpolation.cc:50-137 for the sample points, :252-474 for the
+// method; kdtree2 from src/kdtree/kdtree2.cpp for the neighbour searches).
+//
+// The reference's three loops and what replaces them:
+//   1. per particle: the N nearest particles (itself included: kdtree2 skips
+//      a point only when |i - centre| < correltime, never with correltime 0,
+//      kdtree2.cpp:708-711), h_i = sqrt(d_N)/2, and the particle density
+//      d_i = 1/2 sum m W(r, h_i) scattered both ways (:300-322);
+//   2. per sample point: the N nearest particles, h = sqrt(d_N)/2, and
+//      1/2 sum m W(r, h) over them (:334-389);
+//   3. per particle: every sample point within 2 h_i gets 1/2 m W(r, h_i)
+//      (:391-428), then velocity and intensive scalars are divided by the
+//      density and the density by rho_bar (:431-457).
+// Here both scatters become gathers, so nothing is written twice and no
+// atomics are needed: a sample point sums over the particles whose ball
+// holds it, a particle over the particles whose N-nearest list holds it.
+// Both searches run on one tree: particles sorted by 30-bit Morton code,
+// leaves of 16 consecutive particles, an implicit complete binary tree over
+// the leaves (node k has children 2k, 2k+1) with three sets of boxes: plain
+// (nearest-neighbour pruning), each particle inflated by 2 h_i (sample
+// gathers) and by its N-th neighbour distance (particle-density gather).
+// Walks are stackless: one bit per level records a pending far child.
+//
+// Arithmetic follows the reference: squared distances in float summed over
+// the axes in order (kdtree2's own float), h, the kernel bin and the
+// weights in Real (float or double), the kernel from a 1001-entry table, no
+// FMA. The N-nearest list orders by (squared distance, particle index), so
+// among particles tied at the N-th distance the smaller index is taken;
+// kdtree2 takes whichever its traversal reaches last. Sums over a neighbour
+// list run in ascending distance as the reference's sorted results do; the
+// ball gathers run in tree order where the reference runs in particle
+// order, so those sums agree to rounding.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <vector>
+
+#include "context.h"
+
+namespace dtfe_sph {
+
+constexpr int kBlock = 256;
+constexpr int kLeaf = 16;
+constexpr int kTable = 1000;         // table bins over [0, 2] (SPH_interpolation.cc:282-290)
+constexpr int kBoxParts = 512;       // partial bounding boxes
+constexpr int kMaxScalars = 4;
+constexpr double kPi = 3.14159265;   // src/define.h:47
+
+struct Node {
+    float4 lo, hi;
+};
+
+__device__ __forceinline__ float dist2(float3 q, float4 p) {
+#pragma clang fp contract(off)
+    float a = p.x - q.x, b = p.y - q.y, c = p.z - q.z;
+    float s = a * a;
+    s = s + b * b;
+    s = s + c * c;
+    return s;
+}
+
+// Squared distance from q to a box, never above dist2 to a point inside it
+// (monotone rounding); +inf for an empty box.
+__device__ __forceinline__ float box_dist2(const Node &n, float3 q) {
+#pragma clang fp contract(off)
+    float a = fmaxf(fmaxf(n.lo.x - q.x, q.x - n.hi.x), 0.0f);
+    float b = fmaxf(fmaxf(n.lo.y - q.y, q.y - n.hi.y), 0.0f);
+    float c = fmaxf(fmaxf(n.lo.z - q.z, q.z - n.hi.z), 0.0f);
+    float s = a * a;
+    s = s + b * b;
+    s = s + c * c;
+    return s;
+}
+
+__device__ __forceinline__ bool inside(const Node &n, float3 q) {
+    return q.x >= n.lo.x && q.x <= n.hi.x && q.y >= n.lo.y && q.y <= n.hi.y && q.z >= n.lo.z && q.z <= n.hi.z;
+}
+
+__device__ __forceinline__ uint64_t make_key(float d2, uint32_t idx) {
+    return ((uint64_t)__float_as_uint(d2) << 32) | idx;
+}
+
+__device__ __forceinline__ float key_dist2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+// kernel bin of a pair: int(r / (h dx)) (SPH_interpolation.cc:313, :372, :415),
+// at most the last bin; a zero h (coincident points) gives NaN, bin 0
+template <typename F>
+__device__ __forceinline__ int kernel_bin(float d2, F h, F dx) {
+#pragma clang fp contract(off)
+    const F r = (F)sqrtf(d2);
+    const F q = r / (h * dx);
+    return q >= F(kTable) ? kTable : (q >= F(0) ? (int)q : 0);
+}
+
+// The N nearest particles of q as sorted keys in L[NMAX-N .. NMAX-1]
+// (L[0 .. NMAX-N) hold zeros that never move).
+//
+// The search starts from an upper bound of the N-th key: the largest key of
+// N particles next to q in Morton order (any N distinct particles bound it),
+// so pruning is tight from the first leaf and few candidates are inserted.
+template <int NMAX>
+__device__ void nearest(const Node *__restrict__ nodes, int levels, uint32_t n_pad, const float4 *__restrict__ P,
+                        int64_t n, float3 q, int N, int64_t seed, uint64_t (&L)[NMAX]) {
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) L[j] = j < NMAX - N ? 0ull : ~0ull;
+    const float inf = __builtin_huge_valf();
+    int64_t s_lo = seed - N / 2;
+    s_lo = s_lo < 0 ? 0 : (s_lo + N > n ? n - N : s_lo);
+    uint64_t bound = 0;
+    for (int64_t s = s_lo; s < s_lo + N; ++s) {
+        const float4 p = P[s];
+        const uint64_t key = make_key(dist2(q, p), __float_as_uint(p.w));
+        bound = key > bound ? key : bound;
+    }
+    float wf = key_dist2(bound);
+    uint32_t k = 1, trail = 0;
+    int depth = 0;
+    while (true) {
+        if (depth == levels) {
+            const int64_t s0 = (int64_t)(k - n_pad) * kLeaf;
+            const int64_t s1 = s0 + kLeaf < n ? s0 + kLeaf : n;
+            for (int64_t s = s0; s < s1; ++s) {
+                const float4 p = P[s];
+                const uint64_t key = make_key(dist2(q, p), __float_as_uint(p.w));
+                if (key < L[NMAX - 1] && key <= bound) {
+#pragma unroll
+                    for (int j = NMAX - 1; j > 0; --j) {
+                        const uint64_t prev = L[j - 1];
+                        L[j] = key < L[j] ? (key < prev ? prev : key) : L[j];
+                    }
+                    L[0] = key < L[0] ? key : L[0];  // a full list (N == NMAX) has no zero in front
+                }
+            }
+            wf = key_dist2(L[NMAX - 1] <= bound ? L[NMAX - 1] : bound);
+        } else {
+            const Node a = nodes[2 * k], b = nodes[2 * k + 1];
+            const float da = box_dist2(a, q), db = box_dist2(b, q);
+            const bool a_near = da <= db;
+            const float dn = a_near ? da : db, df = a_near ? db : da;
+            if (dn <= wf && dn < inf) {
+                trail = (trail << 1) | (df <= wf && df < inf ? 1u : 0u);
+                ++depth;
+                k = a_near ? 2 * k : 2 * k + 1;
+                continue;
+            }
+        }
+        while (true) {  // back up to the deepest pending far child still in range
+            if (depth == 0) return;
+            if (trail & 1u) {
+                trail &= ~1u;
+                k ^= 1u;
+                const float d = box_dist2(nodes[k], q);
+                if (d <= wf && d < inf) break;
+                continue;
+            }
+            trail >>= 1;
+            k >>= 1;
+            --depth;
+        }
+    }
+}
+
+// Calls leaf(first, end) for every leaf whose box holds q.
+template <class Leaf>
+__device__ void containing(const Node *__restrict__ nodes, int levels, uint32_t n_pad, int64_t n, float3 q,
+                           Leaf &&leaf) {
+    if (!inside(nodes[1], q)) return;
+    uint32_t k = 1, trail = 0;
+    int depth = 0;
+    while (true) {
+        if (depth == levels) {
+            const int64_t s0 = (int64_t)(k - n_pad) * kLeaf;
+            leaf(s0, s0 + kLeaf < n ? s0 + kLeaf : n);
+        } else {
+            const bool a = inside(nodes[2 * k], q), b = inside(nodes[2 * k + 1], q);
+            if (a || b) {
+                trail = (trail << 1) | (a && b ? 1u : 0u);
+                ++depth;
+                k = a ? 2 * k : 2 * k + 1;
+                continue;
+            }
+        }
+        while (true) {
+            if (depth == 0) return;
+            if (trail & 1u) {
+                trail &= ~1u;
+                k ^= 1u;
+                break;
+            }
+            trail >>= 1;
+            k >>= 1;
+            --depth;
+        }
+    }
+}
+
+// ---- tree build ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_load(const T *__restrict__ pos, int64_t n, float4 *__restrict__ pf, float *__restrict__ part) {
+    __shared__ float red[6][kBlock];
+    const float inf = __builtin_huge_valf();
+    float lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const float x = (float)pos[3 * i], y = (float)pos[3 * i + 1], z = (float)pos[3 * i + 2];
+        pf[i] = make_float4(x, y, z, __uint_as_float((uint32_t)i));
+        lo[0] = fminf(lo[0], x), lo[1] = fminf(lo[1], y), lo[2] = fminf(lo[2], z);
+        hi[0] = fmaxf(hi[0], x), hi[1] = fmaxf(hi[1], y), hi[2] = fmaxf(hi[2], z);
+    }
+    for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = lo[a], red[3 + a][threadIdx.x] = hi[a];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 3; ++a) {
+                red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + w]);
+                red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// The partial boxes of k_load reduced to one: bbox[0..2] low, [3..5] high.
+__global__ void k_bbox(const float *__restrict__ part, int n_parts, float *__restrict__ bbox) {
+    const int a = threadIdx.x;
+    if (a >= 6) return;
+    float v = part[a];
+    for (int b = 1; b < n_parts; ++b) v = a < 3 ? fminf(v, part[6 * b + a]) : fmaxf(v, part[6 * b + a]);
+    bbox[a] = v;
+}
+
+// 30-bit Morton code of a point in the particles' bounding box (clamped).
+__device__ __forceinline__ uint32_t morton(const float *__restrict__ bbox, float3 p) {
+    const float c[3] = {p.x, p.y, p.z};
+    uint32_t m = 0;
+    for (int a = 0; a < 3; ++a) {
+        const float ext = bbox[3 + a] - bbox[a];
+        const float t = ext > 0.0f ? (c[a] - bbox[a]) / ext * 1024.0f : 0.0f;
+        const uint32_t u = t <= 0.0f ? 0u : (t >= 1023.0f ? 1023u : (uint32_t)t);
+        m |= spread10(u) << (2 - a);
+    }
+    return m;
+}
+
+__global__ void k_morton(const float4 *__restrict__ pf, int64_t n, const float *__restrict__ bbox,
+                         uint32_t *__restrict__ code, int32_t *__restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pf[i];
+    code[i] = morton(bbox, make_float3(p.x, p.y, p.z));
+    idx[i] = (int32_t)i;
+}
+
+// Sorted particle data; the velocity and scalars are copied in the same order.
+template <typename F>
+__global__ void k_gather(const float4 *__restrict__ pf, const int32_t *__restrict__ order, int64_t n,
+                         const F *__restrict__ w, const F *__restrict__ vel, const F *__restrict__ sc, int ns,
+                         float4 *__restrict__ P, int32_t *__restrict__ rank, F *__restrict__ ws, F *__restrict__ vs,
+                         F *__restrict__ ss) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const int32_t o = order[s];
+    P[s] = pf[o];
+    rank[o] = (int32_t)s;
+    ws[s] = w[o];
+    if (vel)
+        for (int a = 0; a < 3; ++a) vs[3 * s + a] = vel[3 * (int64_t)o + a];
+    for (int a = 0; a < ns; ++a) ss[(int64_t)ns * s + a] = sc[(int64_t)ns * o + a];
+}
+
+// Leaf boxes: mode 0 plain, 1 inflated by sqrt(r2[s]) (sample gathers),
+// 2 by the N-th neighbour distance (particle-density gather). The margin
+// keeps every point within the radius inside the box despite rounding.
+__global__ void k_leaf_boxes(const float4 *__restrict__ P, int64_t n, uint32_t n_pad, int mode,
+                             const float *__restrict__ r2, const uint64_t *__restrict__ worst, Node *__restrict__ nodes) {
+    const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l >= n_pad) return;
+    const float inf = __builtin_huge_valf();
+    float4 lo = make_float4(inf, inf, inf, 0.0f), hi = make_float4(-inf, -inf, -inf, 0.0f);
+    const int64_t s0 = l * kLeaf, s1 = s0 + kLeaf < n ? s0 + kLeaf : n;
+    for (int64_t s = s0; s < s1; ++s) {
+        const float4 p = P[s];
+        float R = 0.0f;
+        if (mode == 1) R = sqrtf(r2[s]);
+        else if (mode == 2) R = sqrtf(key_dist2(worst[s]));
+        const float c[3] = {p.x, p.y, p.z};
+        float l3[3], h3[3];
+        for (int a = 0; a < 3; ++a) {
+            const float m = mode ? R * 1.0001f + (fabsf(c[a]) + R) * 1e-6f + 1e-30f : 0.0f;
+            l3[a] = c[a] - m, h3[a] = c[a] + m;
+        }
+        lo.x = fminf(lo.x, l3[0]), lo.y = fminf(lo.y, l3[1]), lo.z = fminf(lo.z, l3[2]);
+        hi.x = fmaxf(hi.x, h3[0]), hi.y = fmaxf(hi.y, h3[1]), hi.z = fmaxf(hi.z, h3[2]);
+    }
+    nodes[n_pad + l] = Node{lo, hi};
+}
+
+__global__ void k_merge_boxes(Node *__restrict__ nodes, uint32_t first, uint32_t count) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t k = first + i;
+    const Node a = nodes[2 * k], b = nodes[2 * k + 1];
+    nodes[k] = Node{make_float4(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z), 0.0f),
+                    make_float4(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z), 0.0f)};
+}
+
+// ---- the three passes -----------------------------------------------------------------------
+
+struct Tree {
+    const Node *plain, *ball, *knn_ball;
+    int levels;
+    uint32_t n_pad;
+    int64_t n;
+    const uint32_t *codes;  // sorted Morton codes of the particles
+    const float *bbox;      // their bounding box
+};
+
+// first sorted particle whose Morton code is not below q's
+__device__ __forceinline__ int64_t morton_rank(const Tree &t, float3 q) {
+    const uint32_t key = morton(t.bbox, q);
+    int64_t lo = 0, hi = t.n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (t.codes[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename F>
+struct Parts {
+    const float4 *P;
+    const int32_t *rank;
+    const F *w, *vel, *sc;
+    int ns;
+    F *h, *hf, *c1p, *d;
+    float *r2;
+    uint64_t *worst;
+    const F *W;  // kernel table [kTable + 1]
+    F dx;
+};
+
+// Loop 1 without its scatter: h_i, the per-particle constants, and the
+// particle density's own-list half (SPH_interpolation.cc:300-322).
+template <typename F, int NMAX>
+__global__ void __launch_bounds__(kBlock) k_particles(Tree t, Parts<F> a, int N, bool want_d) {
+#pragma clang fp contract(off)
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= t.n) return;
+    const float4 p = a.P[s];
+    const float3 q = make_float3(p.x, p.y, p.z);
+    uint64_t L[NMAX];
+    nearest<NMAX>(t.plain, t.levels, t.n_pad, a.P, t.n, q, N, s, L);
+    const F h = (F)sqrtf(key_dist2(L[NMAX - 1])) * F(0.5);
+    const F hf = F(1) / (h * h * h);
+    a.h[s] = h;
+    a.hf[s] = hf;
+    a.c1p[s] = F(0.5) * a.w[s] * hf;
+    a.r2[s] = (float)(F(4) * h * h);
+    a.worst[s] = L[NMAX - 1];
+    if (want_d) {
+        F d = F(0);
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+            if (j < NMAX - N || (uint32_t)L[j] >= (uint64_t)t.n) continue;
+            const int32_t o = a.rank[(uint32_t)L[j]];
+            d += F(0.5) * a.w[o] * hf * a.W[kernel_bin(key_dist2(L[j]), h, a.dx)];
+        }
+        a.d[s] = d;
+    }
+}
+
+// The scatter half of the particle density: particle j takes
+// 1/2 m_i W(r, h_i) from every i whose N-nearest list holds j.
+template <typename F>
+__global__ void __launch_bounds__(kBlock) k_particle_density(Tree t, Parts<F> a) {
+#pragma clang fp contract(off)
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= t.n) return;
+    const float4 p = a.P[j];
+    const float3 q = make_float3(p.x, p.y, p.z);
+    const uint32_t oj = __float_as_uint(p.w);
+    F d = a.d[j];
+    containing(t.knn_ball, t.levels, t.n_pad, t.n, q, [&](int64_t s0, int64_t s1) {
+        for (int64_t s = s0; s < s1; ++s) {
+            const float d2 = dist2(q, a.P[s]);
+            if (make_key(d2, oj) <= a.worst[s])
+                d += F(0.5) * a.w[s] * a.hf[s] * a.W[kernel_bin(d2, a.h[s], a.dx)];
+        }
+    });
+    a.d[j] = d;
+}
+
+
+struct Rows {
+    int kind;              // DTFE_SAMPLE_*
+    int64_t n_rows, n0, n1, n2;
+    const float *ax;       // grid: x[n0], y[n1], z[n2] as the tree's float
+    const void *cone;      // cone, Real: r[n0], sin theta[n1], cos theta[n1], sin psi[n2], cos psi[n2]
+    double origin[3];
+    const double *pts;     // user points [n_rows][3]
+};
+
+// Sample point of a row as the reference forms it (SPH_interpolation.cc:63-133):
+// grid and user points as given, cone points origin + r (sin t cos p,
+// sin t sin p, cos t) in Real; then cast to the tree's float.
+template <typename F>
+__device__ __forceinline__ float3 row_point(const Rows &r, int64_t row) {
+#pragma clang fp contract(off)
+    if (r.kind == DTFE_SAMPLE_POINTS)
+        return make_float3((float)r.pts[3 * row], (float)r.pts[3 * row + 1], (float)r.pts[3 * row + 2]);
+    const int64_t n12 = r.n1 * r.n2;
+    const int64_t i0 = row / n12, rem = row - i0 * n12, i1 = rem / r.n2, i2 = rem - i1 * r.n2;
+    if (r.kind == DTFE_SAMPLE_GRID) return make_float3(r.ax[i0], r.ax[r.n0 + i1], r.ax[r.n0 + r.n1 + i2]);
+    const F *c = (const F *)r.cone;
+    const F rr = c[i0], st = c[r.n0 + i1], ct = c[r.n0 + r.n1 + i1];
+    const F sp = c[r.n0 + 2 * r.n1 + i2], cp = c[r.n0 + 2 * r.n1 + r.n2 + i2];
+    const F rs = rr * st;
+    return make_float3((float)((F)r.origin[0] + rs * cp), (float)((F)r.origin[1] + rs * sp),
+                       (float)((F)r.origin[2] + rr * ct));
+}
+
+struct Out {
+    void *density, *velocity, *scalar;
+};
+
+// Loops 2 and 3 for one sample point, then the final divisions.
+template <typename F, int NMAX>
+__global__ void __launch_bounds__(kBlock) k_samples(Tree t, Parts<F> a, Rows r, int N, bool want_rho, bool extensive,
+                                                    F norm, Out out) {
+#pragma clang fp contract(off)
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    int64_t row;
+    if (r.kind == DTFE_SAMPLE_POINTS) {
+        row = tid;
+        if (row >= r.n_rows) return;
+    } else {
+        // a wave takes a 4x4x4 brick of the grid or cone, so its lanes search one region
+        const int64_t b = tid >> 6;
+        const int l = (int)(tid & 63);
+        const int64_t nb1 = (r.n1 + 3) >> 2, nb2 = (r.n2 + 3) >> 2;
+        const int64_t b0 = b / (nb1 * nb2), rem = b - b0 * nb1 * nb2, b1 = rem / nb2, b2 = rem - b1 * nb2;
+        const int64_t i0 = 4 * b0 + (l >> 4), i1 = 4 * b1 + ((l >> 2) & 3), i2 = 4 * b2 + (l & 3);
+        if (i0 >= r.n0 || i1 >= r.n1 || i2 >= r.n2) return;
+        row = (i0 * r.n1 + i1) * r.n2 + i2;
+    }
+    const float3 q = row_point<F>(r, row);
+    const bool vel = a.vel != nullptr;
+    const int ns = a.ns;
+    uint64_t L[NMAX];
+    nearest<NMAX>(t.plain, t.levels, t.n_pad, a.P, t.n, q, N, morton_rank(t, q), L);
+    const F hg = (F)sqrtf(key_dist2(L[NMAX - 1])) * F(0.5);
+    const F c1 = F(1) / (hg * hg * hg);
+    F rho = F(0), v[3] = {F(0), F(0), F(0)}, s[kMaxScalars];
+#pragma unroll
+    for (int k = 0; k < kMaxScalars; ++k) s[k] = F(0);
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        if (j < NMAX - N || (uint32_t)L[j] >= (uint64_t)t.n) continue;
+        const int32_t o = a.rank[(uint32_t)L[j]];
+        const F t1 = F(0.5) * a.w[o] * c1 * a.W[kernel_bin(key_dist2(L[j]), hg, a.dx)];
+        rho += t1;
+        if (vel)
+            for (int k = 0; k < 3; ++k) v[k] += a.vel[3 * (int64_t)o + k] * t1;
+#pragma unroll
+        for (int k = 0; k < kMaxScalars; ++k)
+            if (k < ns) {
+                const F x = a.sc[(int64_t)ns * o + k] * t1;
+                s[k] += extensive ? x / a.d[o] : x;
+            }
+    }
+    containing(t.ball, t.levels, t.n_pad, t.n, q, [&](int64_t s0, int64_t s1) {
+        for (int64_t p = s0; p < s1; ++p) {
+            const float d2 = dist2(q, a.P[p]);
+            if (d2 <= a.r2[p]) {
+                const F t1 = a.c1p[p] * a.W[kernel_bin(d2, a.h[p], a.dx)];
+                rho += t1;
+                if (vel)
+                    for (int k = 0; k < 3; ++k) v[k] += a.vel[3 * p + k] * t1;
+#pragma unroll
+                for (int k = 0; k < kMaxScalars; ++k)
+                    if (k < ns) {
+                        const F x = a.sc[(int64_t)ns * p + k] * t1;
+                        s[k] += extensive ? x : x / a.d[p];
+                    }
+            }
+        }
+    });
+    if (vel) {
+        F *ov = (F *)out.velocity + 3 * row;
+        for (int k = 0; k < 3; ++k) ov[k] = v[k] / rho;
+    }
+    if (ns) {
+        F *os = (F *)out.scalar + (int64_t)ns * row;
+#pragma unroll
+        for (int k = 0; k < kMaxScalars; ++k)
+            if (k < ns) os[k] = extensive ? s[k] : s[k] / rho;
+    }
+    if (want_rho) ((F *)out.density)[row] = rho * norm;
+}
+
+// ---- host ----------------------------------------------------------------------------------
+
+// 256-byte aligned sub-buffers of one scratch allocation.
+struct Carve {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+template <typename F>
+double kernel_value(F x) {  // SPH_smoothingKernel (SPH_interpolation.cc:142-160), 3D
+    const F factor = F(1. / kPi);
+    if (std::fabs(x) <= 1.) return factor * F(1. - 1.5 * x * x + 0.75 * x * x * x);
+    if (std::fabs(x) <= 2.) {
+        const F t = F(2. - x);
+        return factor * F(0.25 * t * t * t);
+    }
+    return F(0.);
+}
+
+void build_boxes(hipStream_t st, const float4 *P, int64_t n, uint32_t n_pad, int levels, int mode, const float *r2,
+                 const uint64_t *worst, Node *nodes) {
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(grid_for(n_pad, kBlock)), dim3(kBlock), 0, st, P, n, n_pad, mode, r2, worst,
+                       nodes);
+    for (int l = levels - 1; l >= 0; --l) {
+        const uint32_t first = 1u << l;
+        hipLaunchKernelGGL(k_merge_boxes, dim3(grid_for(first, kBlock)), dim3(kBlock), 0, st, nodes, first, first);
+    }
+}
+
+template <typename F>
+int sph(dtfe_ctx *ctx, int64_t np, const F *pos, const F *weight, const F *vel, const F *sc, int ns, int N,
+        bool extensive, unsigned fields, const dtfe_sampling *smp, double rho_bar, dtfe_outputs *out) {
+    int rc;
+    hipStream_t st = ctx->stream;
+    // rows and their coordinate tables (SPH_interpolation.cc:57-133)
+    Rows r{};
+    r.kind = smp->kind;
+    std::vector<float> axf;
+    std::vector<F> cone;
+    if (smp->kind == DTFE_SAMPLE_GRID || smp->kind == DTFE_SAMPLE_CONE) {
+        for (int i = 0; i < 3; ++i)
+            if (smp->n[i] < 1) return fail(ctx, DTFE_ERR_ARGUMENT, "grid sizes must be >= 1");
+        if ((double)smp->n[0] * smp->n[1] * smp->n[2] > 4.0e9)
+            return fail(ctx, DTFE_ERR_ARGUMENT, "more than 4e9 sample points");
+        r.n0 = smp->n[0], r.n1 = smp->n[1], r.n2 = smp->n[2];
+        r.n_rows = r.n0 * r.n1 * r.n2;
+        if (smp->kind == DTFE_SAMPLE_GRID) {
+            F b[6], dy[3];
+            for (int i = 0; i < 6; ++i) b[i] = (F)smp->box[i];
+            for (int i = 0; i < 3; ++i) dy[i] = (b[2 * i + 1] - b[2 * i]) / (F)smp->n[i];
+            for (int ax = 0; ax < 3; ++ax) {
+                F y = (F)((double)b[2 * ax] + 0.5 * (double)dy[ax]);
+                for (int64_t i = 0; i < smp->n[ax]; ++i) {
+                    axf.push_back((float)y);
+                    y += dy[ax];
+                }
+            }
+        } else {
+            // the cone's limits are used as given: the reference's SPH path
+            // passes its degree values to sin / cos unconverted
+            F b[6], dx[3];
+            for (int i = 0; i < 6; ++i) b[i] = (F)smp->cone[i];
+            for (int i = 0; i < 3; ++i) dx[i] = (b[2 * i + 1] - b[2 * i]) / (F)smp->n[i];
+            std::vector<F> ang[3];
+            for (int ax = 0; ax < 3; ++ax) {
+                F y = (F)((double)b[2 * ax] + (double)dx[ax] / 2.);
+                for (int64_t i = 0; i < smp->n[ax]; ++i) {
+                    ang[ax].push_back(y);
+                    y += dx[ax];
+                }
+            }
+            cone.insert(cone.end(), ang[0].begin(), ang[0].end());
+            for (F th : ang[1]) cone.push_back(std::sin(th));
+            for (F th : ang[1]) cone.push_back(std::cos(th));
+            for (F ps : ang[2]) cone.push_back(std::sin(ps));
+            for (F ps : ang[2]) cone.push_back(std::cos(ps));
+            for (int i = 0; i < 3; ++i) r.origin[i] = (double)(F)smp->origin[i];
+        }
+    } else if (smp->kind == DTFE_SAMPLE_POINTS) {
+        if (smp->n_points < 0) return fail(ctx, DTFE_ERR_ARGUMENT, "negative point count");
+        if (smp->n_points > 0 && !smp->points) return fail(ctx, DTFE_ERR_ARGUMENT, "points is NULL");
+        r.n_rows = smp->n_points;
+        r.n0 = r.n1 = r.n2 = 1;
+        r.pts = smp->points;
+    } else {
+        return fail(ctx, DTFE_ERR_ARGUMENT, "unknown sampling kind");
+    }
+    out->n_written = 0;
+    if (r.n_rows == 0) return DTFE_OK;
+    if (((fields & DTFE_FIELD_DENSITY) && !out->density) || (vel && !out->velocity) || (ns && !out->scalar))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "an output array of a requested field is NULL");
+
+    // kernel table (SPH_interpolation.cc:282-290)
+    const F dx = F(2. / kTable);
+    F table[kTable + 1];
+    for (int i = 0; i <= kTable; ++i) table[i] = (F)kernel_value<F>(F(dx * (i + 0.5)));
+
+    // tree shape
+    const int64_t n_leaves = (np + kLeaf - 1) / kLeaf;
+    int levels = 0;
+    while (((int64_t)1 << levels) < n_leaves) ++levels;
+    const uint32_t n_pad = 1u << levels;
+    const int n_parts = std::min(kBoxParts, grid_for(np, kBlock));
+
+    size_t sort_bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                               (int32_t *)nullptr, (int32_t *)nullptr, (int)np, 0, 30, st));
+    const bool want_d = ns > 0;
+    Carve c;
+    const size_t o_pf = c.take(np * sizeof(float4)), o_P = c.take(np * sizeof(float4));
+    const size_t o_k0 = c.take(np * 4), o_k1 = c.take(np * 4), o_v0 = c.take(np * 4), o_v1 = c.take(np * 4);
+    const size_t o_rank = c.take(np * 4), o_w = c.take(np * sizeof(F));
+    const size_t o_vel = c.take(vel ? 3 * np * sizeof(F) : 0), o_sc = c.take((size_t)ns * np * sizeof(F));
+    const size_t o_h = c.take(np * sizeof(F)), o_hf = c.take(np * sizeof(F)), o_c1 = c.take(np * sizeof(F));
+    const size_t o_d = c.take(want_d ? np * sizeof(F) : 0), o_r2 = c.take(np * 4), o_worst = c.take(np * 8);
+    const size_t nodes_bytes = 2 * (size_t)n_pad * sizeof(Node);
+    const size_t o_n0 = c.take(nodes_bytes), o_n1 = c.take(nodes_bytes), o_n2 = c.take(want_d ? nodes_bytes : 0);
+    const size_t o_part = c.take(6 * sizeof(float) * n_parts), o_table = c.take(sizeof(table));
+    const size_t o_bbox = c.take(6 * sizeof(float));
+    const size_t o_ax = c.take(axf.size() * sizeof(float)), o_cone = c.take(cone.size() * sizeof(F));
+    const size_t o_sort = c.take(sort_bytes);
+    if ((rc = ensure(ctx, ctx->sph, c.off))) return rc;
+    char *base = (char *)ctx->sph.p;
+    auto at = [&](size_t o) { return (void *)(base + o); };
+
+    HIP_TRY(hipMemcpyAsync(at(o_table), table, sizeof(table), hipMemcpyHostToDevice, st));
+    if (!axf.empty())
+        HIP_TRY(hipMemcpyAsync(at(o_ax), axf.data(), axf.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    if (!cone.empty())
+        HIP_TRY(hipMemcpyAsync(at(o_cone), cone.data(), cone.size() * sizeof(F), hipMemcpyHostToDevice, st));
+    r.ax = (const float *)at(o_ax);
+    r.cone = at(o_cone);
+
+    for (int i = 0; i < 4; ++i)
+        if (!ctx->sph_ev[i]) HIP_TRY(hipEventCreate(&ctx->sph_ev[i]));
+    HIP_TRY(hipEventRecord(ctx->sph_ev[0], st));
+
+    // sorted particles and the plain tree
+    float4 *pf = (float4 *)at(o_pf), *P = (float4 *)at(o_P);
+    hipLaunchKernelGGL((k_load<F>), dim3(n_parts), dim3(kBlock), 0, st, pos, np, pf, (float *)at(o_part));
+    hipLaunchKernelGGL(k_bbox, dim3(1), dim3(64), 0, st, (const float *)at(o_part), n_parts, (float *)at(o_bbox));
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, pf, np, (const float *)at(o_bbox),
+                       (uint32_t *)at(o_k0), (int32_t *)at(o_v0));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(at(o_sort), sort_bytes, (uint32_t *)at(o_k0), (uint32_t *)at(o_k1),
+                                               (int32_t *)at(o_v0), (int32_t *)at(o_v1), (int)np, 0, 30, st));
+    hipLaunchKernelGGL((k_gather<F>), dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, pf,
+                       (const int32_t *)at(o_v1), np, weight, vel, sc, ns, P, (int32_t *)at(o_rank), (F *)at(o_w),
+                       (F *)at(o_vel), (F *)at(o_sc));
+    Node *plain = (Node *)at(o_n0), *ball = (Node *)at(o_n1), *kball = (Node *)at(o_n2);
+    build_boxes(st, P, np, n_pad, levels, 0, nullptr, nullptr, plain);
+    HIP_TRY(hipEventRecord(ctx->sph_ev[1], st));
+
+    Tree t{plain, ball, kball, levels, n_pad, np, (const uint32_t *)at(o_k1), (const float *)at(o_bbox)};
+    Parts<F> a{};
+    a.P = P;
+    a.rank = (const int32_t *)at(o_rank);
+    a.w = (const F *)at(o_w);
+    a.vel = vel ? (const F *)at(o_vel) : nullptr;
+    a.sc = ns ? (const F *)at(o_sc) : nullptr;
+    a.ns = ns;
+    a.h = (F *)at(o_h), a.hf = (F *)at(o_hf), a.c1p = (F *)at(o_c1), a.d = want_d ? (F *)at(o_d) : nullptr;
+    a.r2 = (float *)at(o_r2);
+    a.worst = (uint64_t *)at(o_worst);
+    a.W = (const F *)at(o_table);
+    a.dx = dx;
+
+    // loop 1: smoothing lengths (and the particle density for scalars)
+    auto particles = [&](auto nmax) {
+        constexpr int NM = decltype(nmax)::value;
+        hipLaunchKernelGGL((k_particles<F, NM>), dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, t, a, N, want_d);
+    };
+    if (N <= 8) particles(std::integral_constant<int, 8>{});
+    else if (N <= 16) particles(std::integral_constant<int, 16>{});
+    else if (N <= 32) particles(std::integral_constant<int, 32>{});
+    else particles(std::integral_constant<int, 64>{});
+    build_boxes(st, P, np, n_pad, levels, 1, a.r2, nullptr, ball);
+    if (want_d) {
+        build_boxes(st, P, np, n_pad, levels, 2, nullptr, a.worst, kball);
+        hipLaunchKernelGGL((k_particle_density<F>), dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, t, a);
+    }
+    HIP_TRY(hipEventRecord(ctx->sph_ev[2], st));
+
+    // loops 2 and 3 at the sample points
+    Out o{out->density, out->velocity, out->scalar};
+    const F norm = F(1. / rho_bar);
+    const bool want_rho = (fields & DTFE_FIELD_DENSITY) != 0;
+    const int64_t threads = r.kind == DTFE_SAMPLE_POINTS
+                                ? r.n_rows
+                                : 64 * ((r.n0 + 3) / 4) * ((r.n1 + 3) / 4) * ((r.n2 + 3) / 4);
+    if (threads > (int64_t)kBlock * (1 << 20)) return fail(ctx, DTFE_ERR_ARGUMENT, "too many sample points");
+    auto samples = [&](auto nmax) {
+        constexpr int NM = decltype(nmax)::value;
+        hipLaunchKernelGGL((k_samples<F, NM>), dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, st, t, a, r, N,
+                           want_rho, extensive, norm, o);
+    };
+    if (N <= 8) samples(std::integral_constant<int, 8>{});
+    else if (N <= 16) samples(std::integral_constant<int, 16>{});
+    else if (N <= 32) samples(std::integral_constant<int, 32>{});
+    else samples(std::integral_constant<int, 64>{});
+    HIP_TRY(hipEventRecord(ctx->sph_ev[3], st));
+    HIP_TRY(hipGetLastError());
+    ctx->sph_pending = true;
+    out->n_written = r.n_rows;
+    return DTFE_OK;
+}
+
+}  // namespace dtfe_sph
+
+extern "C" {
+
+int dtfe_hip_sph(dtfe_ctx *ctx, int64_t np, int dtype, const void *pos, const void *weight, const void *velocity,
+                 const void *scalar, int n_scalar, int neighbours, int extensive, unsigned fields,
+                 const dtfe_sampling *sampling, double rho_bar, dtfe_outputs *out) {
+    if (!ctx) return DTFE_ERR_ARGUMENT;
+    if (dtype != DTFE_F32 && dtype != DTFE_F64) return fail(ctx, DTFE_ERR_ARGUMENT, "bad dtype");
+    if (!sampling || !out) return fail(ctx, DTFE_ERR_ARGUMENT, "NULL argument");
+    const unsigned allowed = DTFE_FIELD_DENSITY | DTFE_FIELD_VELOCITY | DTFE_FIELD_SCALAR;
+    if (fields & ~allowed)
+        return fail(ctx, DTFE_ERR_ARGUMENT,
+                    "the SPH method computes density, velocity and scalar fields only (no gradients)");
+    if (np < 1 || np > ((int64_t)dtfe_sph::kBlock << 20))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "particle count outside [1, 2^28]");
+    if (!pos || !weight) return fail(ctx, DTFE_ERR_ARGUMENT, "pos or weight is NULL");
+    if (neighbours < 1 || neighbours > 64 || neighbours > np)
+        return fail(ctx, DTFE_ERR_ARGUMENT, "SPH neighbours must be in [1, min(64, particles)]");
+    if (!(rho_bar > 0.0)) return fail(ctx, DTFE_ERR_ARGUMENT, "rho_bar must be > 0");
+    const bool want_vel = (fields & DTFE_FIELD_VELOCITY) != 0, want_sc = (fields & DTFE_FIELD_SCALAR) != 0;
+    if (want_vel && !velocity) return fail(ctx, DTFE_ERR_ARGUMENT, "velocity is NULL");
+    if (want_sc && (n_scalar < 1 || n_scalar > dtfe_sph::kMaxScalars || !scalar))
+        return fail(ctx, DTFE_ERR_ARGUMENT, "scalar fields need 1..4 components and their array");
+    const int ns = want_sc ? n_scalar : 0;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (dtype == DTFE_F64)
+        return dtfe_sph::sph<double>(ctx, np, (const double *)pos, (const double *)weight,
+                                     want_vel ? (const double *)velocity : nullptr,
+                                     ns ? (const double *)scalar : nullptr, ns, neighbours, extensive != 0, fields,
+                                     sampling, rho_bar, out);
+    return dtfe_sph::sph<float>(ctx, np, (const float *)pos, (const float *)weight,
+                                want_vel ? (const float *)velocity : nullptr, ns ? (const float *)scalar : nullptr, ns,
+                                neighbours, extensive != 0, fields, sampling, rho_bar, out);
+}
+
+int dtfe_hip_last_sph_timings(dtfe_ctx *ctx, double ms[3]) {
+    if (!ctx || !ms) return DTFE_ERR_ARGUMENT;
+    if (ctx->sph_pending) {
+        HIP_TRY(hipEventSynchronize(ctx->sph_ev[3]));
+        for (int i = 0; i < 3; ++i) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, ctx->sph_ev[i], ctx->sph_ev[i + 1]));
+            ctx->sph_ms[i] = t;
+        }
+        ctx->sph_pending = false;
+    }
+    for (int i = 0; i < 3; ++i) ms[i] = ctx->sph_ms[i];
+    return DTFE_OK;
+}
+
+}  // extern "C"

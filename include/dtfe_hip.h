@@ -180,6 +180,25 @@ int dtfe_hip_deposit(dtfe_ctx *ctx, int scheme, int64_t np, const double *pos, i
                      const void *weight, const void *velocity, const double box[6],
                      const int64_t n[3], double rho_bar, unsigned fields, dtfe_outputs *out);
 
+/* SPH interpolation, replacing SPH_interpolation (src/SPH_interpolation.cc:50-474)
+ * with its kdtree2 searches (src/kdtree/kdtree2.cpp). pos [np][3], weight [np],
+ * velocity [np][3], scalar [np][n_scalar] are device arrays of `dtype` (the
+ * reference's Real; neighbour distances are float, as kdtree2's). Each particle
+ * and sample point takes its `neighbours` nearest particles (a particle counts
+ * itself); fields are DTFE_FIELD_DENSITY / _VELOCITY / _SCALAR only, the
+ * reference's SPH having no gradients. `extensive` selects the reference's
+ * extensive scalar weighting. Samples: a grid over box[6], a cone (whose limits
+ * the reference's SPH path passes to sin/cos unconverted: they are used as
+ * given), or user points (every point gives a row; no region filter). Outputs
+ * density [n], velocity [n][3], scalar [n][n_scalar] of `dtype`. At most 64
+ * neighbours and 4 scalars. */
+int dtfe_hip_sph(dtfe_ctx *ctx, int64_t np, int dtype, const void *pos, const void *weight, const void *velocity,
+                 const void *scalar, int n_scalar, int neighbours, int extensive, unsigned fields,
+                 const dtfe_sampling *sampling, double rho_bar, dtfe_outputs *out);
+/* Milliseconds of the last dtfe_hip_sph call: tree build, particle pass,
+ * sample pass (waits for the call to finish). */
+int dtfe_hip_last_sph_timings(dtfe_ctx *ctx, double ms[3]);
+
 /* ---- adjoint (no counterpart in the reference) ------------------------------------------
  * Reverse mode of dtfe_hip_vertex_density + dtfe_hip_sample at fixed
  * tessellation: given dL/d(output rows), returns dL/d(vertex positions,

@@ -978,3 +978,239 @@ double oracle_average_density(int64_t np, const double *weight) {
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------------------
+// SPH interpolation, SPH_interpolation (src/SPH_interpolation.cc:50-474), in
+// both Real = float and Real = double. The kd-tree (src/kdtree/kdtree2.cpp)
+// is replaced by brute force with the same float squared distances
+// (kdtree2.cpp:672-705: sum over the axes of (data - query)^2 from 0) and
+// the same membership rules: the N nearest with the query particle itself
+// among them (correltime 0 excludes nothing, kdtree2.cpp:708-711), and
+// dis <= r2 for the fixed ball (kdtree2.cpp:790-800). Ties at the N-th
+// distance take the smaller particle index. Loop order and every rounding
+// step follow the reference.
+// ----------------------------------------------------------------------------
+namespace {
+
+constexpr int kSphTable = 1000;
+
+template <typename Real>
+Real sph_kernel(Real x) {  // SPH_smoothingKernel, 3D (:142-160)
+    Real factor = Real(1. / 3.14159265);
+    if (std::fabs(x) <= 1.) return factor * Real(1. - 1.5 * x * x + 0.75 * x * x * x);
+    else if (std::fabs(x) <= 2.) {
+        Real temp = Real(2. - x);
+        return factor * Real(0.25 * temp * temp * temp);
+    }
+    return Real(0.);
+}
+
+inline float sph_dis(const float *data, const float *qv) {
+    float dis = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        float t = data[k] - qv[k];
+        dis += t * t;
+    }
+    return dis;
+}
+
+struct SphHit {
+    float dis;
+    int32_t idx;
+};
+
+// the N nearest of qv among pts, ascending by (dis, idx)
+void sph_nearest(int64_t np, const float *pts, const float *qv, int N, std::vector<SphHit> &out) {
+    std::vector<SphHit> all(np);
+    for (int64_t i = 0; i < np; ++i) all[i] = SphHit{sph_dis(pts + 3 * i, qv), (int32_t)i};
+    auto less = [](const SphHit &a, const SphHit &b) { return a.dis < b.dis || (a.dis == b.dis && a.idx < b.idx); };
+    std::partial_sort(all.begin(), all.begin() + N, all.end(), less);
+    out.assign(all.begin(), all.begin() + N);
+}
+
+// Sample points as float (:57-133): grid over box, cone (limits used as
+// given), or user points.
+template <typename Real>
+std::vector<float> sph_points(int kind, const double *boxd, const int64_t *grid, const double *coned,
+                              const double *origind, int64_t npts, const double *pts) {
+    std::vector<float> g;
+    if (kind == 2) {
+        for (int64_t i = 0; i < 3 * npts; ++i) g.push_back((float)(Real)pts[i]);
+        return g;
+    }
+    if (kind == 0) {
+        Real box[6];
+        for (int i = 0; i < 6; ++i) box[i] = (Real)boxd[i];
+        Real y[3], dy[3];
+        for (int i = 0; i < 3; ++i) dy[i] = (box[2 * i + 1] - box[2 * i]) / grid[i];
+        y[0] = box[0] + 0.5 * dy[0];
+        for (int64_t i1 = 0; i1 < grid[0]; ++i1) {
+            y[1] = box[2] + 0.5 * dy[1];
+            for (int64_t i2 = 0; i2 < grid[1]; ++i2) {
+                y[2] = box[4] + 0.5 * dy[2];
+                for (int64_t i3 = 0; i3 < grid[2]; ++i3) {
+                    for (int j = 0; j < 3; ++j) g.push_back((float)y[j]);
+                    y[2] += dy[2];
+                }
+                y[1] += dy[1];
+            }
+            y[0] += dy[0];
+        }
+        return g;
+    }
+    Real box[6], origin[3], dx[3];
+    for (int i = 0; i < 6; ++i) box[i] = (Real)coned[i];
+    for (int i = 0; i < 3; ++i) origin[i] = (Real)origind[i];
+    for (int i = 0; i < 3; ++i) dx[i] = (box[2 * i + 1] - box[2 * i]) / grid[i];
+    Real r = box[0] + dx[0] / 2.;
+    for (int64_t i = 0; i < grid[0]; ++i) {
+        Real theta = box[2] + dx[1] / 2.;
+        for (int64_t j = 0; j < grid[1]; ++j) {
+            Real phi = box[4] + dx[2] / 2.;
+            for (int64_t k = 0; k < grid[2]; ++k) {
+                g.push_back((float)(origin[0] + r * std::sin(theta) * std::cos(phi)));
+                g.push_back((float)(origin[1] + r * std::sin(theta) * std::sin(phi)));
+                g.push_back((float)(origin[2] + r * std::cos(theta)));
+                phi += dx[2];
+            }
+            theta += dx[1];
+        }
+        r += dx[0];
+    }
+    return g;
+}
+
+template <typename Real>
+int64_t sph_run(int64_t np, const Real *pos, const Real *weight, const Real *vel, const Real *sc, int ns, int N,
+                int extensive, int kind, const double *box, const int64_t *grid, const double *cone,
+                const double *origin, int64_t npts, const double *pts, double averageDensity, int want_vel,
+                Real *o_density, Real *o_velocity, Real *o_scalar, Real *o_h, Real *o_d) {
+    std::vector<float> data(3 * np);
+    for (int64_t i = 0; i < 3 * np; ++i) data[i] = (float)pos[i];
+    const std::vector<float> gp = sph_points<Real>(kind, box, grid, cone, origin, npts, pts);
+    const int64_t totalGrid = (int64_t)gp.size() / 3;
+
+    Real dx = Real(2. / kSphTable);
+    Real W[kSphTable + 1];
+    for (int i = 0; i <= kSphTable; ++i) W[i] = sph_kernel<Real>(dx * (i + 0.5));
+    // int(r / (h dx)) (:313, :372, :415); a zero h (a neighbour list of
+    // coincident points) gives NaN, which the reference turns into an
+    // undefined table index: bin 0 here, and at most the last bin
+    auto bin_of = [&](Real temp, Real h) {
+        Real q = temp / (h * dx);
+        return q >= Real(kSphTable) ? kSphTable : (q >= Real(0) ? int(q) : 0);
+    };
+
+    // loop 1 (:300-322)
+    std::vector<Real> h(np, Real(0.)), d(np, Real(0.));
+    std::vector<SphHit> result;
+    for (int64_t i = 0; i < np; ++i) {
+        sph_nearest(np, data.data(), data.data() + 3 * i, N, result);
+        h[i] = Real(std::sqrt(result[N - 1].dis) / 2.);
+        Real const c1 = Real(1.) / (h[i] * h[i] * h[i]);
+        for (size_t j = 0; j < result.size(); ++j) {
+            Real const temp = std::sqrt(result[j].dis);
+            int const id = result[j].idx;
+            int bin1 = bin_of(temp, h[i]);
+            d[i] += Real(0.5) * weight[id] * c1 * W[bin1];
+            d[id] += Real(0.5) * weight[i] * c1 * W[bin1];
+        }
+    }
+
+    // loop 2 (:334-389)
+    std::vector<Real> dens(totalGrid), velo(3 * totalGrid), scal((size_t)ns * totalGrid);
+    for (int64_t g = 0; g < totalGrid; ++g) {
+        sph_nearest(np, data.data(), gp.data() + 3 * g, N, result);
+        Real const tempH = Real(std::sqrt(result[N - 1].dis) / 2.);
+        Real const c1 = Real(1.) / (tempH * tempH * tempH);
+        Real resDens = Real(0.), resVel[3] = {0, 0, 0}, resInt[8] = {0}, resExt[8] = {0};
+        for (size_t j = 0; j < result.size(); ++j) {
+            int const id = result[j].idx;
+            Real tempR = std::sqrt(result[j].dis);
+            int bin1 = bin_of(tempR, tempH);
+            Real temp1 = Real(0.5) * weight[id] * c1 * W[bin1];
+            resDens += temp1;
+            for (int k = 0; k < 3; ++k) resVel[k] += (vel ? vel[3 * id + k] : Real(0)) * temp1;
+            for (int k = 0; k < ns; ++k) {
+                resInt[k] += sc[(int64_t)ns * id + k] * temp1;
+                resExt[k] += sc[(int64_t)ns * id + k] * temp1 / d[id];
+            }
+        }
+        dens[g] = resDens;
+        for (int k = 0; k < 3; ++k) velo[3 * g + k] = resVel[k];
+        for (int k = 0; k < ns; ++k) scal[(size_t)ns * g + k] = extensive ? resExt[k] : resInt[k];
+    }
+
+    // loop 3 (:391-428)
+    for (int64_t i = 0; i < np; ++i) {
+        const float *y = data.data() + 3 * i;
+        Real const distance = Real(4.) * h[i] * h[i];
+        const float r2 = (float)distance;
+        Real const c1 = Real(0.5) * weight[i] * (Real(1.) / (h[i] * h[i] * h[i]));
+        for (int64_t g = 0; g < totalGrid; ++g) {
+            const float dis = sph_dis(gp.data() + 3 * g, y);
+            if (dis > r2) continue;
+            Real tempR = std::sqrt(dis);
+            int bin1 = bin_of(tempR, h[i]);
+            Real temp1 = c1 * W[bin1];
+            dens[g] += temp1;
+            if (want_vel)
+                for (int k = 0; k < 3; ++k) velo[3 * g + k] += vel[3 * i + k] * temp1;
+            for (int k = 0; k < ns; ++k) {
+                if (extensive) scal[(size_t)ns * g + k] += sc[(int64_t)ns * i + k] * temp1;
+                else scal[(size_t)ns * g + k] += sc[(int64_t)ns * i + k] * temp1 / d[i];
+            }
+        }
+    }
+
+    // :431-457
+    if (want_vel)
+        for (int64_t g = 0; g < totalGrid; ++g)
+            for (int k = 0; k < 3; ++k) velo[3 * g + k] /= dens[g];
+    if (ns && !extensive)
+        for (int64_t g = 0; g < totalGrid; ++g)
+            for (int k = 0; k < ns; ++k) scal[(size_t)ns * g + k] /= dens[g];
+    Real factor = Real(1. / averageDensity);
+    for (int64_t g = 0; g < totalGrid; ++g) dens[g] *= factor;
+
+    if (o_density) std::copy(dens.begin(), dens.end(), o_density);
+    if (o_velocity && want_vel) std::copy(velo.begin(), velo.end(), o_velocity);
+    if (o_scalar && ns) std::copy(scal.begin(), scal.end(), o_scalar);
+    if (o_h) std::copy(h.begin(), h.end(), o_h);
+    if (o_d) std::copy(d.begin(), d.end(), o_d);
+    return totalGrid;
+}
+
+}  // namespace
+
+extern "C" {
+
+double oracle_sph_kernel(int real_double, double x) {
+    return real_double ? sph_kernel<double>(x) : (double)sph_kernel<float>((float)x);
+}
+
+// Sample points only (float [n][3]); returns n, writes when out is given.
+int64_t oracle_sph_points(int real_double, int kind, const double *box, const int64_t *grid, const double *cone,
+                          const double *origin, int64_t npts, const double *pts, float *out) {
+    std::vector<float> g = real_double ? sph_points<double>(kind, box, grid, cone, origin, npts, pts)
+                                       : sph_points<float>(kind, box, grid, cone, origin, npts, pts);
+    if (out) std::copy(g.begin(), g.end(), out);
+    return (int64_t)g.size() / 3;
+}
+
+// Arrays are of Real (float when real_double == 0). Returns the row count.
+int64_t oracle_sph(int real_double, int64_t np, const void *pos, const void *weight, const void *vel, const void *sc,
+                   int ns, int N, int extensive, int kind, const double *box, const int64_t *grid,
+                   const double *cone, const double *origin, int64_t npts, const double *pts, double rho_bar,
+                   void *o_density, void *o_velocity, void *o_scalar, void *o_h, void *o_d) {
+    if (real_double)
+        return sph_run<double>(np, (const double *)pos, (const double *)weight, (const double *)vel,
+                               (const double *)sc, ns, N, extensive, kind, box, grid, cone, origin, npts, pts, rho_bar,
+                               vel != nullptr, (double *)o_density, (double *)o_velocity, (double *)o_scalar,
+                               (double *)o_h, (double *)o_d);
+    return sph_run<float>(np, (const float *)pos, (const float *)weight, (const float *)vel, (const float *)sc, ns, N,
+                          extensive, kind, box, grid, cone, origin, npts, pts, rho_bar, vel != nullptr,
+                          (float *)o_density, (float *)o_velocity, (float *)o_scalar, (float *)o_h, (float *)o_d);
+}
+
+}  // extern "C"

@@ -53,6 +53,12 @@ def load():
     lib.oracle_periodic_images.argtypes = [i64, P, P, P, i64, P, P]
     lib.oracle_periodic_images.restype = i64
     lib.oracle_average_density.argtypes = [i64, P]
+    lib.oracle_sph_kernel.argtypes = [i32, d]
+    lib.oracle_sph_kernel.restype = d
+    lib.oracle_sph_points.argtypes = [i32, i32, P, P, P, P, i64, P, P]
+    lib.oracle_sph_points.restype = i64
+    lib.oracle_sph.argtypes = [i32, i64, P, P, P, P, i32, i32, i32, i32, P, P, P, P, i64, P, d, P, P, P, P, P]
+    lib.oracle_sph.restype = i64
     lib.oracle_average_density.restype = d
     _lib = lib
     return lib
@@ -270,3 +276,64 @@ def periodic_images(pos, full_box, padded_box):
 def average_density(weight):
     w = _f64(weight)
     return float(load().oracle_average_density(w.shape[0], _p(w)))
+
+
+_SPH_KINDS = {"grid": 0, "cone": 1, "points": 2}
+
+
+def sph_kernel(x: float, real=np.float64) -> float:
+    """SPH_smoothingKernel in Real (src/SPH_interpolation.cc:142-160)."""
+    return float(load().oracle_sph_kernel(int(real == np.float64), float(x)))
+
+
+def _sph_spec(kind, box, n, cone, origin, points):
+    b = _f64(box if box is not None else np.zeros(6))
+    nn = np.asarray(n if n is not None else (1, 1, 1), dtype=np.int64)
+    cn = _f64(cone if cone is not None else np.zeros(6))
+    og = _f64(origin if origin is not None else np.zeros(3))
+    pts = _f64(points) if points is not None else np.zeros((0, 3))
+    return _SPH_KINDS[kind], b, nn, cn, og, pts
+
+
+def sph_points(kind="grid", box=None, n=None, cone=None, origin=None, points=None, real=np.float64):
+    """The float sample points of the SPH path (src/SPH_interpolation.cc:57-133)."""
+    lib = load()
+    k, b, nn, cn, og, pts = _sph_spec(kind, box, n, cone, origin, points)
+    rd = int(real == np.float64)
+    m = lib.oracle_sph_points(rd, k, _p(b), _p(nn), _p(cn), _p(og), pts.shape[0], _p(pts), None)
+    out = np.empty((m, 3), dtype=np.float32)
+    lib.oracle_sph_points(rd, k, _p(b), _p(nn), _p(cn), _p(og), pts.shape[0], _p(pts), _p(out))
+    return out
+
+
+def sph(pos, weight, vel=None, scalar=None, neighbours=8, rho_bar=1.0, kind="grid", box=None, n=None, cone=None,
+        origin=None, points=None, extensive=False, real=np.float64):
+    """SPH interpolation in Real = float32 or float64, brute-force neighbours
+    (src/SPH_interpolation.cc:252-474). Returns density, velocity, scalar at
+    the rows and the particle smoothing lengths h and densities d."""
+    lib = load()
+    R = np.dtype(real)
+    pos = np.ascontiguousarray(pos, dtype=R)
+    weight = np.ascontiguousarray(weight, dtype=R)
+    vel = None if vel is None else np.ascontiguousarray(vel, dtype=R)
+    scalar = None if scalar is None else np.ascontiguousarray(scalar, dtype=R)
+    ns = 0 if scalar is None else scalar.shape[1]
+    k, b, nn, cn, og, pts = _sph_spec(kind, box, n, cone, origin, points)
+    rd = int(R == np.float64)
+    m = lib.oracle_sph_points(rd, k, _p(b), _p(nn), _p(cn), _p(og), pts.shape[0], _p(pts), None)
+    np_ = pos.shape[0]
+    dens = np.empty(m, dtype=R)
+    v = np.empty((m, 3), dtype=R)
+    sc = np.empty((m, max(ns, 1)), dtype=R)
+    h = np.empty(np_, dtype=R)
+    d = np.empty(np_, dtype=R)
+    lib.oracle_sph(rd, np_, _p(pos), _p(weight), _p(vel) if vel is not None else None,
+                   _p(scalar) if scalar is not None else None, ns, int(neighbours), int(bool(extensive)), k, _p(b),
+                   _p(nn), _p(cn), _p(og), pts.shape[0], _p(pts), float(rho_bar), _p(dens), _p(v), _p(sc), _p(h),
+                   _p(d))
+    out = {"density": dens, "h": h, "d": d}
+    if vel is not None:
+        out["velocity"] = v
+    if ns:
+        out["scalar"] = sc[:, :ns]
+    return out

@@ -138,3 +138,24 @@ def test_dropin_method1(gpu, tmp_path):
         err = np.max(np.abs(got - ref[f])) / np.max(np.abs(ref[f]))
         print(f"drop-in method 1 {f}: {err:.3e}")
         assert err < 1e-10, f
+
+
+def test_dropin_sph(gpu, tmp_path):
+    """SPH through DTFE(): the driver's periodic padding (the reference pads
+    for SPH as for DTFE), then the GPU neighbour searches, against the oracle
+    on the same padded particles."""
+    prob = P.clustered_problem()
+    ps = prob.ps
+    parts = np.concatenate([ps.pos, ps.weight[:, None], ps.vel], 1)
+    box = " ".join(str(x) for x in ps.box)
+    _run(tmp_path, [f"box {box}", "grid 10 10 10", "periodic 1", "fields_a density,velocity", "SPH 24"], parts,
+         real="double")
+    prep = M.prepare_particles(ps)
+    iof = prep.image_of
+    ref = O.sph(prep.pos, ps.weight[iof], ps.vel[iof], None, 24, prep.rho_bar, kind="grid", box=ps.box,
+                n=(10, 10, 10))
+    for f, k in (("density", 1), ("velocity", 3)):
+        got = _load(tmp_path, "a", f, np.float64, k)
+        err = np.max(np.abs(got - ref[f])) / np.max(np.abs(ref[f]))
+        print(f"drop-in SPH {f}: {err:.3e}")
+        assert err < 1e-10, f

@@ -1,0 +1,124 @@
+"""SPH on the device against the oracle (Real = float and double, grid,
+cone and user points, extensive and intensive scalars, 7 to 64
+neighbours), ties on a lattice, the argument checks, and at scale the
+float path against the double path and a constant velocity reproduced."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from dtfe_amd import synthetic as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = 20.0
+FIELDS = ("density", "velocity", "scalar")
+
+
+def _particles(n, seed, ns=2):
+    ps = S.clustered(n, L, 12, 0.3, 1.5, seed, with_velocity=True, lognormal_weights=True)
+    sc = np.random.default_rng(seed + 1).standard_normal((n, ns))
+    return ps.pos, ps.weight, ps.vel, sc
+
+
+def _spec(kind, rng):
+    from dtfe_amd import api
+    if kind == "grid":
+        return api.GridSpec((0.0, L, 0.0, L, 0.0, L), (12, 10, 8)), dict(kind="grid", box=(0, L, 0, L, 0, L),
+                                                                         n=(12, 10, 8))
+    if kind == "cone":
+        cone, n, origin = (1.0, 18.0, 0.2, 1.2, 0.1, 1.3), (6, 5, 4), (0.5, 0.5, 0.5)
+        return api.ConeSpec(cone, n, origin), dict(kind="cone", cone=cone, n=n, origin=origin)
+    pts = rng.random((500, 3)) * L
+    return api.PointsSpec(torch.from_numpy(pts).cuda(), None, (0, L, 0, L, 0, L)), dict(kind="points", points=pts)
+
+
+def _check(got, exp, dtype, label):
+    g = got.double().cpu().numpy().reshape(exp.shape)
+    e = exp.astype(np.float64)
+    if dtype == torch.float64:
+        tol = 1e-11 * (np.abs(e) + np.abs(e).max())
+    else:
+        tol = 2e-5 * np.abs(e) + 2e-5 * np.abs(e).max()
+    err = np.abs(g - e)
+    assert np.all(err <= tol), f"{label}: max err {err.max():.3e} at {np.argmax(err - tol)}"
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("kind", ["grid", "cone", "points"])
+@pytest.mark.parametrize("N,extensive", [(7, False), (32, True), (64, False)])
+def test_sph_matches_oracle(gpu, dtype, kind, N, extensive):
+    from dtfe_amd import api
+    pos, w, vel, sc = _particles(3000, 21 + N)
+    rng = np.random.default_rng(N)
+    spec, ospec = _spec(kind, rng)
+    rho_bar = float(w.sum()) / L ** 3
+    t = lambda a: torch.from_numpy(a).to(gpu, dtype)
+    got = api.sph(t(pos), t(w), t(vel), t(sc), spec, N, rho_bar, FIELDS, extensive)
+    real = np.float64 if dtype == torch.float64 else np.float32
+    exp = O.sph(pos, w, vel, sc, N, rho_bar, extensive=extensive, real=real, **ospec)
+    for f in FIELDS:
+        _check(got[f], exp[f], dtype, f)
+
+
+def test_sph_lattice_ties(gpu):
+    """N = 10 on a cubic lattice ties 12 edge neighbours at sqrt 2 for every
+    interior particle and lattice sample point: the same three are taken."""
+    from dtfe_amd import api
+    g = np.arange(10.0)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(5)
+    w = rng.random(len(pos)) + 0.5
+    sc = rng.standard_normal((len(pos), 1))
+    vel = rng.standard_normal((len(pos), 3))
+    pts = np.stack(np.meshgrid(g[2:8], g[2:8], g[3:5], indexing="ij"), -1).reshape(-1, 3)
+    spec = api.PointsSpec(torch.from_numpy(pts).cuda(), None, (0, 9, 0, 9, 0, 9))
+    t = lambda a: torch.from_numpy(a).to(gpu, torch.float64)
+    for extensive in (False, True):
+        got = api.sph(t(pos), t(w), t(vel), t(sc), spec, 10, 1.0, FIELDS, extensive)
+        exp = O.sph(pos, w, vel, sc, 10, 1.0, kind="points", points=pts, extensive=extensive)
+        for f in FIELDS:
+            _check(got[f], exp[f], torch.float64, f)
+
+
+def test_sph_arguments(gpu):
+    from dtfe_amd import _native as N, api
+    pos, w, vel, sc = _particles(200, 3)
+    t = lambda a: torch.from_numpy(a).to(gpu, torch.float32)
+    spec = api.GridSpec((0.0, L, 0.0, L, 0.0, L), (4, 4, 4))
+    with pytest.raises(N.DTFEError):
+        api.sph(t(pos), t(w), t(vel), None, spec, 65, 1.0, ("density",))
+    with pytest.raises(N.DTFEError):
+        api.sph(t(pos), t(w), t(vel), None, spec, 201, 1.0, ("density",))
+    with pytest.raises(N.DTFEError):
+        api.sph(t(pos), t(w), t(vel), None, spec, 8, 1.0, ("density", "velocity_gradient"))
+    empty = api.PointsSpec(torch.empty((0, 3), dtype=torch.float64, device=gpu), None, (0, L, 0, L, 0, L))
+    assert api.sph(t(pos), t(w), t(vel), None, empty, 8, 1.0, ("density",))["density"].shape == (0,)
+
+
+def test_sph_at_scale(gpu):
+    """The clustered 128^3 particle set on a 64^3 grid, 32 neighbours: the
+    float path within 3e-3 of the double path, and a constant velocity field
+    comes back unchanged. Both paths share the float neighbour distances;
+    int(r / (h dx)) in float or double can fall in adjacent bins of the
+    1000-bin kernel table, a relative step of up to about 1e-3 per term,
+    as between the reference's float and double builds."""
+    from dtfe_amd import api
+    ps = S.clustered_velocity()
+    spec = api.GridSpec(ps.box, (64, 64, 64))
+    rho_bar = float(ps.weight.sum()) / (ps.box[1] - ps.box[0]) ** 3
+    p64 = torch.from_numpy(ps.pos).to(gpu)
+    w64 = torch.from_numpy(ps.weight).to(gpu)
+    v64 = torch.from_numpy(ps.vel).to(gpu)
+    d = api.sph(p64, w64, v64, None, spec, 32, rho_bar, ("density", "velocity"))
+    f = api.sph(p64.float(), w64.float(), v64.float(), None, spec, 32, rho_bar, ("density", "velocity"))
+    rel = ((f["density"].double() - d["density"]).abs() / d["density"]).max().item()
+    print(f"SPH at scale: float vs double density {rel:.2e}, ms {api.last_sph_timings(gpu)}")
+    assert rel < 3e-3
+    vscale = d["velocity"].abs().max().item()
+    assert (f["velocity"].double() - d["velocity"]).abs().max().item() < 3e-3 * vscale
+    const = torch.tensor([3.0, -1.0, 0.25], dtype=torch.float64, device=gpu).expand_as(v64).contiguous()
+    c = api.sph(p64, w64, const, None, spec, 32, rho_bar, ("velocity",))
+    assert (c["velocity"] - const[:1]).abs().max().item() < 1e-12

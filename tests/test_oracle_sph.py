@@ -1,0 +1,119 @@
+"""The SPH oracle pinned without the reference: the kernel table's
+normalisation and continuity, the smoothing lengths against scipy's kd-tree
+(the N nearest counting the particle itself), the sample points of a grid
+and of a cone (limits used as given), and closed forms on a cubic lattice
+for the density, a constant velocity, and the reference's extensive and
+intensive scalar weightings (src/SPH_interpolation.cc:252-474)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from oracle import oracle as O
+
+PI_REF = 3.14159265   # src/define.h:47
+
+
+@pytest.mark.parametrize("real", [np.float64, np.float32])
+def test_kernel_table_normalised_and_continuous(real):
+    dx = 2.0 / 1000
+    r = (np.arange(1000) + 0.5) * dx
+    w = np.array([O.sph_kernel(x, real) for x in r])
+    # midpoint rule of 4 pi r^2 W over [0, 2]: pi / PI_REF, the reference's pi
+    tol = 1e-9 if real == np.float64 else 1e-6
+    assert abs((4 * np.pi * r ** 2 * w * dx).sum() - np.pi / PI_REF) < tol + 2e-6 * (real == np.float32)
+    at1 = 0.25 / PI_REF
+    assert abs(O.sph_kernel(1.0, real) - at1) < 1e-7 * at1
+    assert abs(O.sph_kernel(np.nextafter(1.0, 2.0), real) - at1) < 1e-6 * at1
+    assert O.sph_kernel(2.0, real) == 0.0 and O.sph_kernel(2.5, real) == 0.0
+    assert abs(O.sph_kernel(0.0, real) - 1 / PI_REF) < 1e-7 / PI_REF
+
+
+@pytest.mark.parametrize("N", [2, 8, 33])
+def test_smoothing_lengths_match_kdtree(N):
+    rng = np.random.default_rng(11)
+    p = rng.random((2500, 3)) * 10
+    out = O.sph(p, rng.random(2500) + 0.5, None, None, N, 1.0, kind="grid", box=(0, 10, 0, 10, 0, 10), n=(2, 2, 2))
+    pf = p.astype(np.float32).astype(np.float64)
+    d, _ = cKDTree(pf).query(pf, k=N)
+    d = d.reshape(len(p), -1)[:, N - 1]
+    # h = sqrt(d_N)/2 with d_N the N-th distance counting the particle itself
+    assert np.abs(out["h"] - d / 2).max() <= 1e-6 * d.max()
+
+
+def test_sample_points_grid_and_cone():
+    g = O.sph_points("grid", box=(0, 10, -2, 2, 0, 1), n=(5, 2, 4))
+    assert g.shape == (40, 3)
+    assert np.array_equal(np.unique(g[:, 0]), [1, 3, 5, 7, 9])
+    assert np.array_equal(np.unique(g[:, 1]), [-1, 1])
+    assert np.array_equal(np.unique(g[:, 2]), [0.125, 0.375, 0.625, 0.875])
+    assert np.array_equal(g[:4, 2], [0.125, 0.375, 0.625, 0.875])       # last axis fastest
+    # the cone's angles go to sin / cos as given: (1, 2) and (0, 1) give 1.5 and 0.5
+    c = O.sph_points("cone", cone=(1, 3, 1, 2, 0, 1), n=(1, 1, 1), origin=(5, 6, 7))
+    exp = [5 + 2 * np.sin(1.5) * np.cos(0.5), 6 + 2 * np.sin(1.5) * np.sin(0.5), 7 + 2 * np.cos(1.5)]
+    assert np.allclose(c[0], exp, rtol=0, atol=1e-6)
+    p = np.array([[1.5, 2.5, 3.5], [0.1, 0.2, 0.3]])
+    assert np.array_equal(O.sph_points("points", points=p), p.astype(np.float32))
+
+
+def _lattice(m):
+    g = np.arange(10.0)
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    return pos, np.full(len(pos), m)
+
+
+@pytest.mark.parametrize("real", [np.float64, np.float32])
+@pytest.mark.parametrize("mass", [1.0, 2.5])
+def test_lattice_closed_forms(real, mass):
+    """Unit cubic lattice, N = 7 (itself and the 6 face neighbours, so
+    h = 1/2), sample points on interior sites. Face neighbours sit at r = 2h,
+    the last table bin, so only the particle on the site counts, twice (its
+    own list and its ball): density = 2 * 1/2 m W0 / h^3 = 8 m W0 with
+    W0 = W(0.001); the particle density d is the same. The reference's
+    extensive scalar divides by d in the list half only, the intensive one in
+    the ball half only, then by the density."""
+    pos, w = _lattice(mass)
+    rho_bar = 3.0
+    s = np.full((len(pos), 2), [2.0, -5.0])
+    vel = np.tile([1.0, -2.0, 0.5], (len(pos), 1))
+    pts = np.array([[4.0, 4.0, 4.0], [5.0, 4.0, 5.0], [4.0, 5.0, 5.0]])
+    W0 = O.sph_kernel(0.001, real)
+    rho = 8 * mass * W0
+    tol = 1e-8 if real == np.float64 else 2e-6
+    for extensive in (False, True):
+        o = O.sph(pos, w, vel, s, 7, rho_bar, kind="points", points=pts, extensive=extensive, real=real)
+        assert np.allclose(o["density"], rho / rho_bar, rtol=tol, atol=0)
+        assert np.allclose(o["velocity"], [1.0, -2.0, 0.5], rtol=tol, atol=0)
+        centre = (4 * 10 + 4) * 10 + 4
+        assert o["h"][centre] == 0.5 and abs(o["d"][centre] - rho) <= tol * rho
+        half = 0.5 * mass * 8 * W0          # one particle's 1/2 m W0 / h^3
+        num = s[0] * (half / rho + half)    # list half / d plus ball half, or the other way round
+        exp = num if extensive else num / rho
+        assert np.allclose(o["scalar"], exp, rtol=tol, atol=0), (extensive, o["scalar"][0], exp)
+
+
+def test_ties_take_the_smaller_index():
+    """N = 10 on the lattice: itself, 6 faces and 3 of the 12 edge
+    neighbours tied at sqrt 2 (the three of smallest index)."""
+    pos, _ = _lattice(1.0)
+    rng = np.random.default_rng(3)
+    w = rng.random(len(pos)) + 0.5
+    pt = np.array([[4.0, 4.0, 4.0]])
+    o = O.sph(pos, w, None, None, 10, 1.0, kind="points", points=pt)
+    d2 = ((pos - pt) ** 2).sum(1)
+    order = np.lexsort((np.arange(len(pos)), d2))[:10]
+    h = float(np.sqrt(np.float32(2.0))) / 2      # from kdtree2's float squared distance
+    table = lambda r, hh: O.sph_kernel((int(np.float64(r) / (hh * 0.002)) + 0.5) * 0.002)
+    c1 = 1 / h ** 3
+    own = sum(0.5 * w[j] * c1 * table(np.sqrt(d2[j]), h) for j in order)
+    assert np.isclose(own, o["density"][0] - _ball_part(pos, w, o["h"], pt[0]), rtol=1e-12)
+
+
+def _ball_part(pos, w, h, q):
+    tot = 0.0
+    for i in range(len(pos)):
+        d2 = float(((pos[i] - q) ** 2).sum())
+        if d2 <= 4 * h[i] * h[i]:
+            tot += 0.5 * w[i] / h[i] ** 3 * O.sph_kernel((int(np.sqrt(d2) / (h[i] * 0.002)) + 0.5) * 0.002)
+    return tot
