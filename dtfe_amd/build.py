@@ -75,6 +75,7 @@ def build_dropin(verbose: bool = True) -> None:
     for real, lib in DROPIN_LIBS.items():
         defs = ["-DDOUBLE"] if real == "double" else []
         cmd = [cxx, *common, *defs, "-shared", "-o", lib + ".tmp", os.path.join(DROPIN, "DTFE.cpp"),
+               os.path.join(DROPIN, "input_output.cpp"),
                f"-L{HERE}", "-l:libdtfe_hip.so", "-l:libdtfe_host.so", f"-L{rocm}/lib", "-lamdhip64",
                "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{rocm}/lib"]
         if verbose:

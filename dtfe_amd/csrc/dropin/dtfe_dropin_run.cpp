@@ -4,7 +4,8 @@
 //   dtfe_dropin_run <options.txt> <particles.bin> [<samples.bin>] <out_prefix>
 //
 // options.txt: "key value..." lines (box, grid, periodic, fields_u, fields_a,
-// method, points, seed, cone, origin, region, CIC, TSC, SPH n, extensive, density0);
+// method, points, seed, cone, origin, region, CIC, TSC, SPH n, extensive, density0,
+// output <type> <name> (writeOutputData), program <command line>);
 // particles.bin: n x 7 doubles (x, y, z, weight, vx, vy, vz);
 // samples.bin: m x 6 doubles (x, y, z, dx, dy, dz).
 // Each output field F of uQuantities/aQuantities goes to <out_prefix>.{u,a}.F
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "DTFE.h"
+#include "input_output.h"
 
 static std::vector<double> read_doubles(const std::string &path) {
     std::ifstream f(path, std::ios::binary);
@@ -61,6 +63,28 @@ static void dump_all(const std::string &prefix, const Quantities &q) {
     dump(prefix + ".scalar_gradient", q.scalar_gradient);
 }
 
+static Real synthetic_value(size_t i, size_t c, size_t j) { return Real(double(i * c + j) * 0.125 - 3.0 + 1.0 / 3.0); }
+
+template <typename T>
+static void fill_rows(std::vector<T> &v, size_t n, size_t c) {
+    v.resize(n);
+    Real *p = reinterpret_cast<Real *>(v.data());
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < c; ++j) p[i * c + j] = synthetic_value(i, c, j);
+}
+
+static void fill_synthetic(Quantities &q, const Field &f, size_t n) {
+    if (f.density) fill_rows(q.density, n, 1);
+    if (f.velocity) fill_rows(q.velocity, n, noVelComp);
+    if (f.velocity_gradient) fill_rows(q.velocity_gradient, n, noGradComp);
+    if (f.velocity_divergence) fill_rows(q.velocity_divergence, n, 1);
+    if (f.velocity_shear) fill_rows(q.velocity_shear, n, noShearComp);
+    if (f.velocity_vorticity) fill_rows(q.velocity_vorticity, n, noVortComp);
+    if (f.velocity_std) fill_rows(q.velocity_std, n, 1);
+    if (f.scalar) fill_rows(q.scalar, n, noScalarComp);
+    if (f.scalar_gradient) fill_rows(q.scalar_gradient, n, noScalarGradComp);
+}
+
 int main(int argc, char **argv) {
     if (argc < 4) {
         std::fprintf(stderr, "usage: %s options.txt particles.bin [samples.bin] out_prefix\n", argv[0]);
@@ -68,6 +92,7 @@ int main(int argc, char **argv) {
     }
     User_options o;
     o.verboseLevel = 0;
+    size_t synthetic_rows = 0;  // > 0: no DTFE() call; the requested fields get known values, then are written
     std::ifstream opt(argv[1]);
     std::string line;
     while (std::getline(opt, line)) {
@@ -85,10 +110,15 @@ int main(int argc, char **argv) {
         else if (k == "cone") { o.redshiftConeOn = true; for (int i = 0; i < 6; ++i) ss >> o.redshiftCone[i]; }
         else if (k == "origin") { o.originPosition.assign(3, 0); for (int i = 0; i < 3; ++i) ss >> o.originPosition[i]; }
         else if (k == "region") { o.regionOn = true; for (int i = 0; i < 6; ++i) ss >> o.region[i]; }
-        else if (k == "CIC") o.CIC = true;
-        else if (k == "TSC") o.TSC = true;
-        else if (k == "SPH") { o.SPH = true; ss >> o.SPH_neighbors; }
+        // as the reference's option parser (src/user_options.cc:791-806), a
+        // comparison method turns DTFE off
+        else if (k == "CIC") o.CIC = true, o.DTFE = false;
+        else if (k == "TSC") o.TSC = true, o.DTFE = false;
+        else if (k == "SPH") { o.SPH = true, o.DTFE = false; ss >> o.SPH_neighbors; }
         else if (k == "extensive") o.extensive = true;
+        else if (k == "output") ss >> o.outputFileType >> o.outputFilename;
+        else if (k == "program") std::getline(ss >> std::ws, o.programOptions);
+        else if (k == "synthetic_rows") ss >> synthetic_rows;
         else if (k == "density0") ss >> o.averageDensity;
         else if (k == "value_formula") { int v; ss >> v; o.referenceValueFormula = v != 0; }
     }
@@ -112,6 +142,21 @@ int main(int argc, char **argv) {
     }
     dtfe_dropin::throw_on_error(true);
     Quantities uq, aq;
+    if (synthetic_rows) {
+        // row i, component j of a field with c components: (i * c + j) * 0.125 - 3.0 + 1/3
+        if (!o.regionOn) o.region = o.boxCoordinates;
+        o.userDefinedSampling = argc >= 5;
+        fill_synthetic(uq, o.uField, synthetic_rows);
+        fill_synthetic(aq, o.aField, synthetic_rows);
+        try {
+            writeOutputData(uq, aq, o);
+        } catch (const dtfe_dropin::Error &e) {
+            std::fprintf(stderr, "output error: %s\n", e.what());
+            return 1;
+        }
+        std::printf("ok synthetic\n");
+        return 0;
+    }
     try {
         DTFE(&parts, samples, o, &uq, &aq);
     } catch (const dtfe_dropin::Error &e) {
@@ -120,6 +165,14 @@ int main(int argc, char **argv) {
     }
     dump_all(prefix + ".u", uq);
     dump_all(prefix + ".a", aq);
+    if (o.outputFileType) {
+        try {
+            writeOutputData(uq, aq, o);
+        } catch (const dtfe_dropin::Error &e) {
+            std::fprintf(stderr, "output error: %s\n", e.what());
+            return 1;
+        }
+    }
     std::printf("ok particles_left=%zu averageDensity=%.17g tessellation_s=%.3f\n", parts.size(),
                 (double)o.averageDensity, dtfe_dropin::last_tessellation_seconds());
     return 0;

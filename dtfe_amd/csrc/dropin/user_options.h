@@ -77,6 +77,7 @@ struct Box {
 
 struct User_options {
     std::string inputFilename, outputFilename;
+    std::string programOptions;  // the command line, copied into the header of output format 100
     std::vector<size_t> gridSize;
     bool periodic;
     Box boxCoordinates;

@@ -148,8 +148,11 @@ def test_dropin_sph(gpu, tmp_path):
     ps = prob.ps
     parts = np.concatenate([ps.pos, ps.weight[:, None], ps.vel], 1)
     box = " ".join(str(x) for x in ps.box)
-    _run(tmp_path, [f"box {box}", "grid 10 10 10", "periodic 1", "fields_a density,velocity", "SPH 24"], parts,
-         real="double")
+    _run(tmp_path, [f"box {box}", "grid 10 10 10", "periodic 1", "fields_a density,velocity", "SPH 24",
+                    f"output 101 {tmp_path / 'file'}"], parts, real="double")
+    # writeOutputData's raw format holds the same values as the fields
+    assert (tmp_path / "file.a_den").read_bytes() == (tmp_path / "out.a.density").read_bytes()
+    assert (tmp_path / "file.a_vel").read_bytes() == (tmp_path / "out.a.velocity").read_bytes()
     prep = M.prepare_particles(ps)
     iof = prep.image_of
     ref = O.sph(prep.pos, ps.weight[iof], ps.vel[iof], None, 24, prep.rho_bar, kind="grid", box=ps.box,
