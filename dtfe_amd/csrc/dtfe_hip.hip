@@ -1707,14 +1707,76 @@ __global__ void k_seed_fill(const int *__restrict__ in, int *__restrict__ out, i
 // ----------------------------------------------------------------------------
 // user-point region filter (points outside `box` produce no row)
 // ----------------------------------------------------------------------------
-__global__ void k_points_in_box(const double *__restrict__ pts, int64_t n, const double *__restrict__ box,
-                                int *__restrict__ flag) {
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double *q = pts + 3 * i;
-        flag[i] = (q[0] >= box[0] && q[0] <= box[1] && q[1] >= box[2] && q[1] <= box[3] && q[2] >= box[4] &&
-                   q[2] <= box[5]) ? 1 : 0;
+// A stream compaction on the device: per-tile counts, one scan of the
+// counts, then each tile writes the indices of its points inside the box in
+// input order. Only the row count goes back to the host.
+constexpr int kRegionPer = 4;                       // points per thread
+constexpr int kRegionTile = kBlock * kRegionPer;    // points per workgroup
+
+__device__ __forceinline__ int point_in_box(const double *__restrict__ pts, int64_t i, int64_t n,
+                                            const double *__restrict__ box) {
+    if (i >= n) return 0;
+    const double *q = pts + 3 * i;
+    return (q[0] >= box[0] && q[0] <= box[1] && q[1] >= box[2] && q[1] <= box[3] && q[2] >= box[4] &&
+            q[2] <= box[5]) ? 1 : 0;
+}
+
+// exclusive prefix sum of v over the workgroup; total in *sum
+__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int *sum) {
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kBlock; d <<= 1) {
+        const int x = t >= d ? lds[t - d] : 0;
+        __syncthreads();
+        lds[t] += x;
+        __syncthreads();
     }
+    const int incl = lds[t];
+    *sum = lds[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_region_count(const double *__restrict__ pts, int64_t n,
+                                                         const double *__restrict__ box, int *__restrict__ counts) {
+    __shared__ int lds[kBlock];
+    const int64_t base = (int64_t)blockIdx.x * kRegionTile + (int64_t)threadIdx.x * kRegionPer;
+    int c = 0;
+    for (int k = 0; k < kRegionPer; ++k) c += point_in_box(pts, base + k, n, box);
+    int total;
+    block_exclusive_scan(c, lds, &total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// one workgroup: offsets[i] = sum of counts[0 .. i), offsets[n_tiles] = total
+__global__ void __launch_bounds__(kBlock) k_region_scan(const int *__restrict__ counts, int64_t n_tiles,
+                                                        int64_t *__restrict__ offsets) {
+    __shared__ int lds[kBlock];
+    int64_t carry = 0;
+    for (int64_t c0 = 0; c0 < n_tiles; c0 += kBlock) {
+        const int64_t i = c0 + threadIdx.x;
+        const int v = i < n_tiles ? counts[i] : 0;
+        int total;
+        const int ex = block_exclusive_scan(v, lds, &total);
+        if (i < n_tiles) offsets[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) offsets[n_tiles] = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_region_write(const double *__restrict__ pts, int64_t n,
+                                                         const double *__restrict__ box,
+                                                         const int64_t *__restrict__ offsets,
+                                                         int64_t *__restrict__ rows) {
+    __shared__ int lds[kBlock];
+    const int64_t base = (int64_t)blockIdx.x * kRegionTile + (int64_t)threadIdx.x * kRegionPer;
+    int f[kRegionPer], c = 0;
+    for (int k = 0; k < kRegionPer; ++k) c += (f[k] = point_in_box(pts, base + k, n, box));
+    int total;
+    int64_t at = offsets[blockIdx.x] + block_exclusive_scan(c, lds, &total);
+    for (int k = 0; k < kRegionPer; ++k)
+        if (f[k]) rows[at++] = base + k;
 }
 
 // ----------------------------------------------------------------------------
@@ -2735,28 +2797,27 @@ int setup_rows(dtfe_ctx *ctx, const dtfe_sampling *smp, int method, SampleArgs &
         a.n[0] = a.n[1] = a.n[2] = 1;
         a.pts = smp->points;
         a.deltas = smp->deltas;
-        // region filter + compaction (host round trip of one flag per point)
-        int64_t np = smp->n_points;
-        const size_t box_off = ((size_t)std::max<int64_t>(np, 1) * sizeof(int) + 255) & ~(size_t)255;
-        if ((rc = ensure(ctx, ctx->flags, box_off + 6 * sizeof(double)))) return rc;
-        double *dbox = (double *)((char *)ctx->flags.p + box_off);
+        // region filter: the points inside box[], in input order, compacted
+        // on the device; the host reads back the count only
+        const int64_t np = smp->n_points;
+        const int64_t n_tiles = (np + kRegionTile - 1) / kRegionTile;
+        if (n_tiles > (1 << 20)) return fail(ctx, DTFE_ERR_ARGUMENT, "more than 2^30 user points");
+        const size_t counts_off = 256, offsets_off = counts_off + (((size_t)n_tiles * sizeof(int) + 255) & ~(size_t)255);
+        if ((rc = ensure(ctx, ctx->flags, offsets_off + (n_tiles + 1) * sizeof(int64_t)))) return rc;
+        double *dbox = (double *)ctx->flags.p;
+        int *counts = (int *)((char *)ctx->flags.p + counts_off);
+        int64_t *offsets = (int64_t *)((char *)ctx->flags.p + offsets_off);
+        if ((rc = ensure(ctx, ctx->rows, std::max<int64_t>(np, 1) * sizeof(int64_t)))) return rc;
         HIP_TRY(hipMemcpyAsync(dbox, smp->box, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        std::vector<int> fl(np);
         if (np > 0) {
-            hipLaunchKernelGGL(k_points_in_box, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, smp->points, np,
-                               dbox, (int *)ctx->flags.p);
-            HIP_TRY(hipMemcpyAsync(fl.data(), ctx->flags.p, np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            hipLaunchKernelGGL(k_region_count, dim3((unsigned)n_tiles), dim3(kBlock), 0, ctx->stream, smp->points, np,
+                               dbox, counts);
+            hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(kBlock), 0, ctx->stream, counts, n_tiles, offsets);
+            hipLaunchKernelGGL(k_region_write, dim3((unsigned)n_tiles), dim3(kBlock), 0, ctx->stream, smp->points, np,
+                               dbox, offsets, (int64_t *)ctx->rows.p);
+            HIP_TRY(hipMemcpyAsync(&n_rows, offsets + n_tiles, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
         }
-        std::vector<int64_t> rows;
-        rows.reserve(np);
-        for (int64_t i = 0; i < np; ++i)
-            if (fl[i]) rows.push_back(i);
-        n_rows = (int64_t)rows.size();
-        if ((rc = ensure(ctx, ctx->rows, std::max<int64_t>(n_rows, 1) * sizeof(int64_t)))) return rc;
-        if (n_rows)
-            HIP_TRY(hipMemcpyAsync(ctx->rows.p, rows.data(), n_rows * sizeof(int64_t), hipMemcpyHostToDevice,
-                                   ctx->stream));
         a.row_point = (const int64_t *)ctx->rows.p;
         a.n[0] = n_rows > 0 ? n_rows : 1;
     } else {
