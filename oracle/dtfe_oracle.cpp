@@ -18,6 +18,9 @@
 //   * CIC / TSC deposits .... src/CIC_interpolation.cc:67-273, src/TSC_interpolation.cc:67-254
 //   * periodic padding ...... src/subpartition.h:79-154
 //   * average density ....... src/subpartition.h:437-449, src/box.h:141-148
+//   * Sobol points, method 1  src/CGAL_triangulation/averaged_interpolation_1.cc:31-48, :196-421
+//   * SPH ................... src/SPH_interpolation.cc:50-474 (Real = float and
+//                             double; brute-force neighbours for kdtree2)
 // Point location is a visibility walk with exact signs (exact.h); like
 // CGAL's locate it starts from the previously located cell.
 #include <cstdint>
