@@ -102,20 +102,33 @@ __device__ __forceinline__ int kernel_bin(float d2, F h, F dx) {
 // The search starts from an upper bound of the N-th key: the largest key of
 // N particles next to q in Morton order (any N distinct particles bound it),
 // so pruning is tight from the first leaf and few candidates are inserted.
+//
+// With `worst` (the particles' own N-th keys, known for sample points), each
+// of those particles p also bounds it by the triangle inequality: the ball
+// of radius |q - p| + r_p around q holds p's N nearest.
 template <int NMAX>
 __device__ void nearest(const Node *__restrict__ nodes, int levels, uint32_t n_pad, const float4 *__restrict__ P,
-                        int64_t n, float3 q, int N, int64_t seed, uint64_t (&L)[NMAX]) {
+                        int64_t n, float3 q, int N, int64_t seed, uint64_t (&L)[NMAX],
+                        const uint64_t *__restrict__ worst = nullptr) {
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) L[j] = j < NMAX - N ? 0ull : ~0ull;
     const float inf = __builtin_huge_valf();
     int64_t s_lo = seed - N / 2;
     s_lo = s_lo < 0 ? 0 : (s_lo + N > n ? n - N : s_lo);
-    uint64_t bound = 0;
+    uint64_t bound = 0, tri = ~0ull;
     for (int64_t s = s_lo; s < s_lo + N; ++s) {
         const float4 p = P[s];
-        const uint64_t key = make_key(dist2(q, p), __float_as_uint(p.w));
+        const float d2 = dist2(q, p);
+        const uint64_t key = make_key(d2, __float_as_uint(p.w));
         bound = key > bound ? key : bound;
+        if (worst) {
+            // (|q - p| + r_p)^2 with a margin far above the roundings of both distances
+            const float t = sqrtf(d2) + sqrtf(key_dist2(worst[s]));
+            const uint64_t tk = make_key(t * t * 1.0001f + 1e-30f, 0xFFFFFFFFu);
+            tri = tk < tri ? tk : tri;
+        }
     }
+    bound = tri < bound ? tri : bound;
     float wf = key_dist2(bound);
     uint32_t k = 1, trail = 0;
     int depth = 0;
@@ -461,7 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_samples(Tree t, Parts<F> a, Rows r, 
     const bool vel = a.vel != nullptr;
     const int ns = a.ns;
     uint64_t L[NMAX];
-    nearest<NMAX>(t.plain, t.levels, t.n_pad, a.P, t.n, q, N, morton_rank(t, q), L);
+    nearest<NMAX>(t.plain, t.levels, t.n_pad, a.P, t.n, q, N, morton_rank(t, q), L, a.worst);
     const F hg = (F)sqrtf(key_dist2(L[NMAX - 1])) * F(0.5);
     const F c1 = F(1) / (hg * hg * hg);
     F rho = F(0), v[3] = {F(0), F(0), F(0)}, s[kMaxScalars];
