@@ -122,3 +122,25 @@ def test_sph_at_scale(gpu):
     const = torch.tensor([3.0, -1.0, 0.25], dtype=torch.float64, device=gpu).expand_as(v64).contiguous()
     c = api.sph(p64, w64, const, None, spec, 32, rho_bar, ("velocity",))
     assert (c["velocity"] - const[:1]).abs().max().item() < 1e-12
+
+
+@pytest.mark.parametrize("extensive", [False, True])
+def test_sph_scalars_linear_at_scale(gpu, extensive):
+    """Scalars on the clustered 128^3 set (the particle-density gather runs):
+    the output is linear in the scalar field for both of the reference's
+    weightings, so a third component 2 s1 - 3 s2 comes back as 2 out1 - 3 out2."""
+    from dtfe_amd import api
+    ps = S.clustered_velocity()
+    rng = np.random.default_rng(9)
+    s12 = rng.standard_normal((ps.n, 2))
+    sc = np.concatenate([s12, (2 * s12[:, 0] - 3 * s12[:, 1])[:, None]], axis=1)
+    spec = api.GridSpec(ps.box, (48, 48, 48))
+    rho_bar = float(ps.weight.sum()) / (ps.box[1] - ps.box[0]) ** 3
+    t = lambda a: torch.from_numpy(a).to(gpu, torch.float64)
+    o = api.sph(t(ps.pos), t(ps.weight), None, t(sc), spec, 32, rho_bar, ("density", "scalar"), extensive)
+    out = o["scalar"]
+    comb = 2 * out[:, 0] - 3 * out[:, 1]
+    scale = out[:, :2].abs().max().item()
+    err = (out[:, 2] - comb).abs().max().item()
+    print(f"SPH scalar linearity ({'extensive' if extensive else 'intensive'}): {err:.2e} of {scale:.2e}")
+    assert err <= 1e-10 * scale
