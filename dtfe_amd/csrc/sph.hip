@@ -366,17 +366,38 @@ struct Parts {
     F dx;
 };
 
+// An upper bound of every particle's N-th key: the largest key of the N
+// particles next to it in Morton order (the start of its own search). The
+// particle pass then also bounds each particle by the triangle inequality
+// through these.
+__global__ void k_window_bound(const float4 *__restrict__ P, int64_t n, int N, uint64_t *__restrict__ wb) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = P[s];
+    const float3 q = make_float3(p.x, p.y, p.z);
+    int64_t s_lo = s - N / 2;
+    s_lo = s_lo < 0 ? 0 : (s_lo + N > n ? n - N : s_lo);
+    uint64_t bound = 0;
+    for (int64_t j = s_lo; j < s_lo + N; ++j) {
+        const float4 o = P[j];
+        const uint64_t key = make_key(dist2(q, o), __float_as_uint(o.w));
+        bound = key > bound ? key : bound;
+    }
+    wb[s] = bound;
+}
+
 // Loop 1 without its scatter: h_i, the per-particle constants, and the
 // particle density's own-list half (SPH_interpolation.cc:300-322).
 template <typename F, int NMAX>
-__global__ void __launch_bounds__(kBlock) k_particles(Tree t, Parts<F> a, int N, bool want_d) {
+__global__ void __launch_bounds__(kBlock) k_particles(Tree t, Parts<F> a, int N, bool want_d,
+                                                      const uint64_t *__restrict__ wb) {
 #pragma clang fp contract(off)
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= t.n) return;
     const float4 p = a.P[s];
     const float3 q = make_float3(p.x, p.y, p.z);
     uint64_t L[NMAX];
-    nearest<NMAX>(t.plain, t.levels, t.n_pad, a.P, t.n, q, N, s, L);
+    nearest<NMAX>(t.plain, t.levels, t.n_pad, a.P, t.n, q, N, s, L, wb);
     const F h = (F)sqrtf(key_dist2(L[NMAX - 1])) * F(0.5);
     const F hf = F(1) / (h * h * h);
     a.h[s] = h;
@@ -644,6 +665,7 @@ int sph(dtfe_ctx *ctx, int64_t np, const F *pos, const F *weight, const F *vel, 
     const size_t o_vel = c.take(vel ? 3 * np * sizeof(F) : 0), o_sc = c.take((size_t)ns * np * sizeof(F));
     const size_t o_h = c.take(np * sizeof(F)), o_hf = c.take(np * sizeof(F)), o_c1 = c.take(np * sizeof(F));
     const size_t o_d = c.take(want_d ? np * sizeof(F) : 0), o_r2 = c.take(np * 4), o_worst = c.take(np * 8);
+    const size_t o_wb = c.take(np * 8);
     const size_t nodes_bytes = 2 * (size_t)n_pad * sizeof(Node);
     const size_t o_n0 = c.take(nodes_bytes), o_n1 = c.take(nodes_bytes), o_n2 = c.take(want_d ? nodes_bytes : 0);
     const size_t o_part = c.take(6 * sizeof(float) * n_parts), o_table = c.take(sizeof(table));
@@ -696,9 +718,12 @@ int sph(dtfe_ctx *ctx, int64_t np, const F *pos, const F *weight, const F *vel, 
     a.dx = dx;
 
     // loop 1: smoothing lengths (and the particle density for scalars)
+    hipLaunchKernelGGL(k_window_bound, dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, (const float4 *)P, np, N,
+                       (uint64_t *)at(o_wb));
     auto particles = [&](auto nmax) {
         constexpr int NM = decltype(nmax)::value;
-        hipLaunchKernelGGL((k_particles<F, NM>), dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, t, a, N, want_d);
+        hipLaunchKernelGGL((k_particles<F, NM>), dim3(grid_for(np, kBlock)), dim3(kBlock), 0, st, t, a, N, want_d,
+                           (const uint64_t *)at(o_wb));
     };
     if (N <= 8) particles(std::integral_constant<int, 8>{});
     else if (N <= 16) particles(std::integral_constant<int, 16>{});
