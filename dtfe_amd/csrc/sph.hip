@@ -118,12 +118,16 @@ __device__ void nearest(const Node *__restrict__ nodes, int levels, uint32_t n_p
     uint64_t bound = 0, tri = ~0ull;
     for (int64_t s = s_lo; s < s_lo + N; ++s) {
         const float4 p = P[s];
-        const float d2 = dist2(q, p);
-        const uint64_t key = make_key(d2, __float_as_uint(p.w));
+        const uint64_t key = make_key(dist2(q, p), __float_as_uint(p.w));
         bound = key > bound ? key : bound;
-        if (worst) {
-            // (|q - p| + r_p)^2 with a margin far above the roundings of both distances
-            const float t = sqrtf(d2) + sqrtf(key_dist2(worst[s]));
+    }
+    if (worst) {
+        // (|q - p| + r_p)^2 over 2N Morton neighbours either side, with a
+        // margin far above the roundings of both distances
+        const int64_t t_lo = seed - 2 * N < 0 ? 0 : seed - 2 * N;
+        const int64_t t_hi = seed + 2 * N > n ? n : seed + 2 * N;
+        for (int64_t s = t_lo; s < t_hi; ++s) {
+            const float t = sqrtf(dist2(q, P[s])) + sqrtf(key_dist2(worst[s]));
             const uint64_t tk = make_key(t * t * 1.0001f + 1e-30f, 0xFFFFFFFFu);
             tri = tk < tri ? tk : tri;
         }
