@@ -477,10 +477,10 @@ struct Out {
 
 // Loops 2 and 3 for one sample point, then the final divisions.
 template <typename F, int NMAX>
-__global__ void __launch_bounds__(kBlock) k_samples(Tree t, Parts<F> a, Rows r, int N, bool want_rho, bool extensive,
-                                                    F norm, Out out) {
+__global__ void __launch_bounds__(kBlock) k_samples(Tree t, Parts<F> a, Rows r, int64_t tid0, int N, bool want_rho,
+                                                    bool extensive, F norm, Out out) {
 #pragma clang fp contract(off)
-    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t tid = tid0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
     int64_t row;
     if (r.kind == DTFE_SAMPLE_POINTS) {
         row = tid;
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_samples(Tree t, Parts<F> a, Rows r, 
         const int64_t nb1 = (r.n1 + 3) >> 2, nb2 = (r.n2 + 3) >> 2;
         const int64_t b0 = b / (nb1 * nb2), rem = b - b0 * nb1 * nb2, b1 = rem / nb2, b2 = rem - b1 * nb2;
         const int64_t i0 = 4 * b0 + (l >> 4), i1 = 4 * b1 + ((l >> 2) & 3), i2 = 4 * b2 + (l & 3);
-        if (i0 >= r.n0 || i1 >= r.n1 || i2 >= r.n2) return;
+        if (b0 >= (r.n0 + 3) >> 2 || i0 >= r.n0 || i1 >= r.n1 || i2 >= r.n2) return;
         row = (i0 * r.n1 + i1) * r.n2 + i2;
     }
     const float3 q = row_point<F>(r, row);
@@ -747,11 +747,15 @@ int sph(dtfe_ctx *ctx, int64_t np, const F *pos, const F *weight, const F *vel, 
     const int64_t threads = r.kind == DTFE_SAMPLE_POINTS
                                 ? r.n_rows
                                 : 64 * ((r.n0 + 3) / 4) * ((r.n1 + 3) / 4) * ((r.n2 + 3) / 4);
-    if (threads > (int64_t)kBlock * (1 << 20)) return fail(ctx, DTFE_ERR_ARGUMENT, "too many sample points");
+    // launches of at most 2^28 threads (2^20 workgroups), each a multiple of 64
+    const int64_t chunk = (int64_t)kBlock << 20;
     auto samples = [&](auto nmax) {
         constexpr int NM = decltype(nmax)::value;
-        hipLaunchKernelGGL((k_samples<F, NM>), dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, st, t, a, r, N,
-                           want_rho, extensive, norm, o);
+        for (int64_t t0 = 0; t0 < threads; t0 += chunk) {
+            const int64_t nt = std::min(chunk, threads - t0);
+            hipLaunchKernelGGL((k_samples<F, NM>), dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                               t, a, r, t0, N, want_rho, extensive, norm, o);
+        }
     };
     if (N <= 8) samples(std::integral_constant<int, 8>{});
     else if (N <= 16) samples(std::integral_constant<int, 16>{});
