@@ -577,6 +577,10 @@ struct SampleArgs {
     double origin[3];
     const double *pts, *deltas;
     const int64_t *row_point;       // user points kept after the region filter
+    // adjoint of Monte-Carlo user points: every row's MT words saved in row
+    // order, and the rows in a spatial order (both NULL elsewhere)
+    const uint32_t *row_words;
+    const int32_t *row_order;
     const double *equi;             // [M][3] equidistant offsets
     const uint32_t *plan;           // MT states, one per workgroup
     int64_t rows_per_block;
@@ -1479,6 +1483,94 @@ __global__ void DTFE_SAMPLE_BOUNDS k_adjoint_mc(SampleArgs a) {
         __syncthreads();
         for (int64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x)
             if (!adjoint_row<F, VEL, false, RingWords<>, LdsSink, KIND>(a, row, words, sink)) defer_row(a, row, words);
+        __syncthreads();
+        sink.template flush<F>(a);
+        __syncthreads();
+    }
+}
+
+// Every row's 3M MT words copied out in row order: the ring walk of
+// k_sample_mc, so that the adjoint of user points can take its rows in any
+// order and still give each row its own words.
+__global__ void DTFE_SAMPLE_BOUNDS k_save_words(SampleArgs a, uint32_t *__restrict__ out) {
+    extern __shared__ uint32_t ring[];
+    const int64_t row0 = (int64_t)blockIdx.x * a.rows_per_block;
+    if (row0 >= a.n_rows) return;
+    const int64_t row_end = min(row0 + a.rows_per_block, a.n_rows);
+    const int64_t wpr = 3 * (int64_t)a.M;
+    const int64_t first_block = (wpr * row0) / MT_N;
+    const int64_t base = first_block - 1;
+    for (int i = threadIdx.x; i < MT_N; i += blockDim.x) ring[i] = a.plan[(int64_t)blockIdx.x * MT_N + i];
+    RingWords<> words{ring, base * MT_N, (uint32_t)a.ring};
+    int64_t have = first_block - 1;
+    for (int64_t r0 = row0; r0 < row_end; r0 += a.rows_per_round) {
+        const int64_t r1 = min(r0 + a.rows_per_round, row_end);
+        const int64_t need = (wpr * r1 - 1) / MT_N;
+        while (have < need) {
+            mt_generate_block(ring, have + 1, base, a.ring);
+            ++have;
+        }
+        __syncthreads();
+        // consecutive threads write consecutive words
+        for (int64_t k = wpr * r0 + threadIdx.x; k < wpr * r1; k += blockDim.x) out[k] = words(k);
+    }
+}
+
+// Spatial order of user points: a counting sort by a 5-bit-per-axis Morton
+// cell of the region box (rows inside one cell in any order).
+constexpr int kOrderBits = 5;
+
+__device__ __forceinline__ int order_cell(const SampleArgs &a, int64_t row, const double *__restrict__ box) {
+    const double *q = a.pts + 3 * a.row_point[row];
+    int c = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double ext = box[2 * d + 1] - box[2 * d];
+        const double t = ext > 0.0 ? (q[d] - box[2 * d]) / ext * (1 << kOrderBits) : 0.0;
+        const int u = t > 0.0 ? (t >= (1 << kOrderBits) - 1 ? (1 << kOrderBits) - 1 : (int)t) : 0;  // NaN: 0
+        for (int bit = 0; bit < kOrderBits; ++bit) c |= ((u >> bit) & 1) << (3 * bit + d);
+    }
+    return c;
+}
+
+__global__ void k_order_count(SampleArgs a, const double *__restrict__ box, int *__restrict__ counts,
+                              int32_t *__restrict__ cell) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < a.n_rows; row += stride) {
+        const int c = order_cell(a, row, box);
+        cell[row] = c;
+        atomicAdd(counts + c, 1);
+    }
+}
+
+__global__ void k_order_scatter(int64_t n_rows, const int32_t *__restrict__ cell, int64_t *__restrict__ offsets,
+                                int32_t *__restrict__ order) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n_rows; row += stride) {
+        const unsigned long long at = atomicAdd((unsigned long long *)(offsets + cell[row]), 1ull);
+        order[at] = (int32_t)row;
+    }
+}
+
+// Backward of Monte-Carlo user points in the spatial order: each workgroup
+// takes 256 neighbouring points, reads their saved words, and merges their
+// per-vertex gradients in LDS before the HBM atomics.
+template <typename F, bool VEL>
+__global__ void DTFE_SAMPLE_BOUNDS k_adjoint_points(SampleArgs a) {
+    __shared__ int keys[kSinkSlots];
+    __shared__ double vals[kSinkSlots * kSinkVals];
+    const LdsSink sink{keys, vals};
+    const int64_t wpr = 3 * (int64_t)a.M;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.n_rows;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        sink.clear();
+        __syncthreads();
+        const int64_t i = base + threadIdx.x;
+        if (i < a.n_rows) {
+            const int64_t row = a.row_order[i];
+            const SavedWords words{a.row_words + row * wpr, wpr * row};
+            if (!adjoint_row<F, VEL, false, SavedWords, LdsSink, DTFE_SAMPLE_POINTS>(a, row, words, sink))
+                defer_row(a, row, words);
+        }
         __syncthreads();
         sink.template flush<F>(a);
         __syncthreads();
@@ -2659,6 +2751,8 @@ void launch_sample(const SampleArgs &a, int which, int mc, int grid, size_t lds,
         hipLaunchKernelGGL((k_sample_exact<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), 0, s, a, n_def);
     else if (which == 2)
         hipLaunchKernelGGL((k_sample_exact_all<F, VEL, NS, ADJ>), dim3(grid), dim3(kBlock), mc ? lds : 0, s, a);
+    else if (mc && ADJ && a.row_order)
+        hipLaunchKernelGGL((k_adjoint_points<F, VEL>), dim3(grid), dim3(kBlock), 0, s, a);
     else if (mc && ADJ) {
         switch (a.kind) {
             case DTFE_SAMPLE_GRID: hipLaunchKernelGGL((k_adjoint_mc<F, VEL, DTFE_SAMPLE_GRID>), dim3(grid), dim3(kBlock), lds, s, a); break;
@@ -2953,6 +3047,36 @@ int launch_rows(dtfe_ctx *ctx, SampleArgs &a, const dtfe_mc *mc, bool f64, bool 
     a.defer_rows = (int64_t *)ctx->defer_rows.p;
     a.defer_words = (uint32_t *)ctx->defer_words.p;
     a.defer_cap = (int)cap;
+    // the adjoint of Monte-Carlo user points takes its rows in a spatial
+    // order (neighbouring points share vertices, merged in LDS), each row
+    // with its own saved MT words
+    a.row_words = nullptr;
+    a.row_order = nullptr;
+    if (adj && is_mc && a.kind == DTFE_SAMPLE_POINTS && !exact_all && n_rows > 0) {
+        const int64_t n_cells = (int64_t)1 << (3 * kOrderBits);
+        auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t o_counts = up((size_t)n_rows * wpr * sizeof(uint32_t));
+        const size_t o_offsets = o_counts + up((size_t)n_cells * sizeof(int));
+        const size_t o_cell = o_offsets + up((size_t)(n_cells + 1) * sizeof(int64_t));
+        const size_t o_order = o_cell + up((size_t)n_rows * sizeof(int32_t));
+        if ((rc = ensure(ctx, ctx->adj_order, o_order + (size_t)n_rows * sizeof(int32_t)))) return rc;
+        char *b = (char *)ctx->adj_order.p;
+        uint32_t *wbuf = (uint32_t *)b;
+        int *counts = (int *)(b + o_counts);
+        int64_t *offsets = (int64_t *)(b + o_offsets);
+        int32_t *cell = (int32_t *)(b + o_cell), *order = (int32_t *)(b + o_order);
+        const double *dbox = (const double *)ctx->flags.p;  // the region box, from setup_rows
+        hipLaunchKernelGGL(k_save_words, dim3(grid), dim3(kBlock), lds, ctx->stream, a, wbuf);
+        hipLaunchKernelGGL(k_fill<int>, dim3(grid_for(n_cells)), dim3(kBlock), 0, ctx->stream, counts, n_cells, 0);
+        hipLaunchKernelGGL(k_order_count, dim3(grid_for(n_rows)), dim3(kBlock), 0, ctx->stream, a, dbox, counts, cell);
+        hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(kBlock), 0, ctx->stream, counts, n_cells, offsets);
+        hipLaunchKernelGGL(k_order_scatter, dim3(grid_for(n_rows)), dim3(kBlock), 0, ctx->stream, n_rows,
+                           (const int32_t *)cell, offsets, order);
+        a.row_words = wbuf;
+        a.row_order = order;
+        grid = grid_for(n_rows);
+        lds = 0;
+    }
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     if (lanes) {
         if ((rc = ensure(ctx, ctx->centre, n_rows * sizeof(int32_t)))) return rc;
@@ -3167,7 +3291,7 @@ int dtfe_hip_destroy(dtfe_ctx *ctx) {
                       &ctx->status, &ctx->axes, &ctx->trig, &ctx->equi, &ctx->flags,
                       &ctx->rows, &ctx->mass, &ctx->mom, &ctx->defer_rows, &ctx->defer_words, &ctx->adj_rho, &ctx->adj_dW,
                       &ctx->adj_dW2, &ctx->adj_partial, &ctx->adj_acc, &ctx->centre, &ctx->records, &ctx->m1_qr,
-                      &ctx->m1_bins, &ctx->m1_list, &ctx->front, &ctx->sph};
+                      &ctx->m1_bins, &ctx->m1_list, &ctx->front, &ctx->sph, &ctx->adj_order};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &pl : ctx->plans)
