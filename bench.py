@@ -251,6 +251,7 @@ def main():
               "workload": "same mesh, grid and fields; averaging method 1 (Sobol points in the tetrahedra)"}
 
     e2e = end_to_end(m, prep, spec, dt, dev, args.samples, mc_seed)
+    dep = deposits(ps, n, dt, dev)
 
     sph = None
     if not args.no_sph:
@@ -295,12 +296,46 @@ def main():
         "front_end_ms": fe,
         "method1": m1,
         "end_to_end_from_host": e2e,
+        "deposits": dep,
         "sph": sph,
         "fwd_adjoint_ratio": None if fa is None else fa["ratio"],
         "fwd_adjoint": fa,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def deposits(ps, n, dt, dev):
+    """CIC and TSC deposits of density and velocity on the same grid: CIC on
+    the particles as given (the reference pads CIC by 0), TSC on their
+    periodic images within one grid cell of the box (its padding). Algorithmic
+    bytes: particles in once (24 + esz + 3 esz each), cells out once (4 esz)."""
+    import torch
+    from dtfe_amd import api, mesh as M
+    box = ps.box
+    rho_bar = M.average_density(ps.weight)
+    L = box[1] - box[0]
+    pad = L / n[0]
+    ipos, iof = M.periodic_images(ps.pos, box, (box[0] - pad, box[1] + pad, box[2] - pad, box[3] + pad,
+                                                box[4] - pad, box[5] + pad))
+    esz = 4 if dt == torch.float32 else 8
+    out = {}
+    for scheme, pos_h, idx in (("cic", ps.pos, np.arange(ps.n)), ("tsc", ipos, iof)):
+        pos = torch.from_numpy(np.ascontiguousarray(pos_h)).to(dev)
+        w = torch.from_numpy(ps.weight[idx]).to(dev, dt)
+        v = torch.from_numpy(ps.vel[idx]).to(dev, dt)
+        for _ in range(2):
+            api.deposit(scheme, pos, w, v, box, n, rho_bar)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            api.deposit(scheme, pos, w, v, box, n, rho_bar)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / 5
+        nbytes = pos.shape[0] * (24 + 4 * esz) + int(np.prod(n)) * 4 * esz
+        out[scheme] = {"ms": ms, "particles": int(pos.shape[0]), "algorithmic_bytes": int(nbytes),
+                       "achieved_GBps": nbytes / (ms * 1e-3) / 1e9}
+    return out
 
 
 def end_to_end(m, prep, spec, dt, dev, samples, mc_seed):
